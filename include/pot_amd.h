@@ -1,0 +1,156 @@
+/* pot_amd.h -- C ABI of the MI355X (gfx950) entropic Sinkhorn library.
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t
+ * (passed as void*), runs stream-ordered on that stream and returns an int:
+ * >= 0 a solver status, < 0 an error.  All buffers are owned by the caller;
+ * scratch space is the caller's `workspace` (its size comes from the matching
+ * *_workspace_size function).  Weights, potentials and error histories are
+ * float64 device arrays whatever the cost dtype.
+ *
+ * The reference (POT 0.9.7.dev0) reaches this arithmetic through its backend
+ * operators (ot/backend.py:1306-1307 logsumexp, nx.dot / nx.einsum); its own
+ * native FFI precedent is EMD_wrap (ot/lp/EMD_wrapper.cpp:209, declared in
+ * ot/lp/EMD.h): caller-allocated float64 outputs plus an int status that the
+ * Python layer maps to warnings (ot/lp/emd_wrap.pyx:30-43).  The conventions
+ * below follow that precedent.
+ */
+#ifndef POT_AMD_H
+#define POT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status and error codes ------------------------------------------ */
+#define POTAMD_CONVERGED 0    /* err < stopThr at a check iteration */
+#define POTAMD_MAX_ITER 1     /* numItermax reached (the reference warns) */
+#define POTAMD_NUMERICAL 2    /* scaling domain: 0 / nan / inf, state reverted */
+#define POTAMD_NEED_EXACT 3   /* log domain: a column mass left the fp32 range */
+#define POTAMD_EINVAL (-1)    /* invalid argument */
+#define POTAMD_EHIP (-2)      /* HIP runtime error */
+#define POTAMD_EUNSUPPORTED (-3) /* shape outside the compiled kernel set */
+#define POTAMD_EWORKSPACE (-4)   /* workspace too small */
+
+#define POTAMD_F32 0
+#define POTAMD_F64 1
+#define POTAMD_LOG 0      /* log-domain potentials (sinkhorn_log, log batch) */
+#define POTAMD_SCALING 1  /* scaling vectors (sinkhorn_knopp, plain batch) */
+#define POTAMD_COL_FIRST 0 /* v update first: ot.sinkhorn / sinkhorn_log / knopp */
+#define POTAMD_ROW_FIRST 1 /* u update first: ot.batch projections */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int potamd_version(void);
+
+/* Human-readable text of a status / error code. */
+const char* potamd_status_string(int code);
+
+/* Bytes of workspace needed by potamd_sinkhorn_dense. */
+size_t potamd_dense_workspace_size(int dtype, int batch, int n, int m, int rows_per_block);
+
+/* Dense-cost Sinkhorn on `batch` problems of shape n x m sharing one reg.
+ *
+ * Replaces the iteration loops of
+ *   ot.bregman.sinkhorn_log       ot/bregman/_sinkhorn.py:690-918  (LOG, COL_FIRST)
+ *   ot.bregman.sinkhorn_knopp     ot/bregman/_sinkhorn.py:487-687  (SCALING, COL_FIRST)
+ *   ot.batch.bregman_log_projection_batch  ot/batch/_utils.py:165-290 (LOG, ROW_FIRST)
+ *   ot.batch.bregman_projection_batch      ot/batch/_utils.py:46-162  (SCALING, ROW_FIRST)
+ * with the reference's check cadence (every 10th iteration), strict
+ * `err < stopThr`, and the batch's single stop on the largest error.
+ *
+ * M: cost (kernel_given = 0; element exponent -M/reg) or the kernel itself
+ *    (kernel_given = 1: log kernel in LOG, K in SCALING), dtype `dtype`,
+ *    problem p row i at M + p*bstride + i*ldm.
+ * a, b: float64 weights, batch x n and batch x m.
+ * u, v: float64, batch x n / batch x m.  On entry the initial state
+ *    (LOG: log potentials, zeros for a cold start; the COL_FIRST solve uses
+ *    only u, v is recomputed first.  SCALING: scalings).  On exit the state
+ *    the reference returns (log_u/log_v, or u/v).
+ * err_norm: how a batch combines its per-problem errors: 0 = maximum (the
+ *    ot.batch stop, ot/batch/_utils.py:273-278), 1 = Frobenius norm (the
+ *    multi-histogram knopp, ot/bregman/_sinkhorn.py:646-652, which runs the
+ *    histograms as a batch sharing M through bstride = 0).
+ * err_hist: float64 device array of err_cap entries, the reference's
+ *    log["err"] (one value per check).
+ * threads, cols_per_thread, rows_per_block: tile shape, 0 = automatic.
+ * host_ctrl: pinned host memory of >= 512 bytes used for asynchronous stop
+ *    polling, or NULL (then each check synchronises).
+ * info (host, 4 ints): [0] niter (the reference's `ii`), [1] number of err
+ *    values, [2] status, [3] iteration of a numerical stop.
+ */
+int potamd_sinkhorn_dense(int dtype, int domain, int order, int kernel_given,
+                          const void* M, int64_t ldm, int64_t bstride, int batch, int n, int m,
+                          double reg, const double* a, const double* b, double* u, double* v,
+                          int max_iter, double stop_thr, int err_norm, double* err_hist,
+                          int err_cap, int threads, int cols_per_thread, int rows_per_block,
+                          void* workspace, size_t ws_bytes, void* host_ctrl, void* stream,
+                          int* info);
+
+/* Bytes of workspace needed by potamd_plan_dense. */
+size_t potamd_plan_workspace_size(int batch, int n);
+
+/* Plan and per-problem sums from potentials (fp64 element arithmetic).
+ *
+ * Replaces the returns of sinkhorn_log (ot/bregman/_sinkhorn.py:915-918),
+ * sinkhorn_knopp (:683-687), the loss of sinkhorn2 (:436-438), the batch
+ * plan / log plan (ot/batch/_utils.py:160, :286-290), loss_linear_batch
+ * (ot/batch/_linear.py:123-147) and the entropy / KL values of solve_batch
+ * (_linear.py:338-345).
+ * P / logP: outputs in `dtype`, contiguous n x m per problem with problem
+ *    stride pstride; either may be NULL.
+ * sums: float64 device, batch x 4: sum M*P, sum P log(P+1e-16),
+ *    sum P log(P/(a b)+1e-16) (needs a, b; else 0), sum P.  May be NULL.
+ */
+int potamd_plan_dense(int dtype, int domain, int kernel_given, const void* M, int64_t ldm,
+                      int64_t bstride, int batch, int n, int m, double reg, const double* u,
+                      const double* v, const double* a, const double* b, void* P, void* logP,
+                      int64_t pstride, double* sums, void* workspace, size_t ws_bytes,
+                      void* stream);
+
+/* Batched (squared) Euclidean cost matrices.
+ * Replaces ot.dist / ot.utils.euclidean_distances (ot/utils.py:270-307,
+ * :465-468; clamp = 1, zero_diag when X is Y) and
+ * ot.batch.dist_euclidean_batch (ot/batch/_linear.py:54-85; clamp = 0).
+ * X: batch x n x d, Y: batch x m x d, M: batch x n x ldm (same dtype).
+ */
+int potamd_sqeuclidean(int dtype, const void* X, const void* Y, int batch, int n, int m, int d,
+                       int64_t xstride, int64_t ystride, void* M, int64_t ldm, int64_t mstride,
+                       int squared, int clamp, int zero_diag, void* stream);
+
+/* Bytes of workspace needed by potamd_sinkhorn_log_batch (total_n = sum of
+ * the problems' row counts). */
+size_t potamd_batch_workspace_size(int64_t total_n);
+
+/* Ragged batch of independent sinkhorn_log problems, one per workgroup.
+ *
+ * Each problem p runs exactly the loop of ot.bregman.sinkhorn_log
+ * (ot/bregman/_sinkhorn.py:880-906) on its own cost with its own reg and
+ * initial u, stopping on its own check (the batch of the reference,
+ * ot/batch/_utils.py:165-290, needs equal shapes, one reg and one stop; this
+ * entry point generalises it for many uneven problems).
+ * M: packed float32 costs; problem p at M + moff[p], n_p x m_p with row stride
+ *    ldms[p].  All descriptor arrays (moff, ns, ms, ldms, aoff, boff, regs,
+ *    order) are device arrays of `batch` entries; max_m >= every m_p (<= 1024).
+ * a, u: problem p's rows at offset aoff[p]; b, v: columns at boff[p].
+ *    u holds u0 on entry (zeros = the reference's cold start).
+ * order: launch order of the problems (e.g. largest first) or NULL.
+ * vec_aligned: 1 if M is 16-byte aligned and every moff, ldms, m_p is a
+ *    multiple of 4 (vector loads), else 0.
+ * niter, status: per-problem outputs (device int arrays); err_last / err_first:
+ *    last and first checked marginal errors (device, may be NULL).
+ */
+int potamd_sinkhorn_log_batch(int dtype, int batch, int max_m, const void* M,
+                              const int64_t* moff, const int* ns, const int* ms, const int* ldms,
+                              const int64_t* aoff, const int64_t* boff, const double* regs,
+                              const double* a, const double* b, double* u, double* v,
+                              int max_iter, double stop_thr, const int* order, int vec_aligned,
+                              int* niter, int* status, double* err_last, double* err_first,
+                              void* workspace, size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POT_AMD_H */

@@ -1,0 +1,92 @@
+"""ctypes binding of ``libpot_amd.so`` (the C ABI declared in include/pot_amd.h).
+
+The library is loaded from the package directory only.  There is no fallback:
+if it is missing or lacks a symbol, every compute entry point raises.
+"""
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpot_amd.so")
+
+c_int = ctypes.c_int
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+c_size = ctypes.c_size_t
+c_vp = ctypes.c_void_p
+c_ip = ctypes.POINTER(ctypes.c_int)
+
+# name -> (restype, argtypes); must match include/pot_amd.h
+SIGNATURES = {
+    "potamd_version": (c_int, []),
+    "potamd_status_string": (ctypes.c_char_p, [c_int]),
+    "potamd_dense_workspace_size": (c_size, [c_int, c_int, c_int, c_int, c_int]),
+    "potamd_sinkhorn_dense": (c_int, [
+        c_int, c_int, c_int, c_int,          # dtype, domain, order, kernel_given
+        c_vp, c_i64, c_i64, c_int, c_int, c_int,  # M, ldm, bstride, batch, n, m
+        c_dbl, c_vp, c_vp, c_vp, c_vp,       # reg, a, b, u, v
+        c_int, c_dbl, c_int, c_vp, c_int,    # max_iter, stop_thr, err_norm, err_hist, err_cap
+        c_int, c_int, c_int,                 # threads, cols_per_thread, rows_per_block
+        c_vp, c_size, c_vp, c_vp, c_ip]),    # workspace, ws_bytes, host_ctrl, stream, info
+    "potamd_plan_workspace_size": (c_size, [c_int, c_int]),
+    "potamd_plan_dense": (c_int, [
+        c_int, c_int, c_int, c_vp, c_i64, c_i64, c_int, c_int, c_int, c_dbl,
+        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_size, c_vp]),
+    "potamd_batch_workspace_size": (c_size, [c_i64]),
+    "potamd_sinkhorn_log_batch": (c_int, [
+        c_int, c_int, c_int, c_vp,           # dtype, batch, max_m, M
+        c_vp, c_vp, c_vp, c_vp,              # moff, ns, ms, ldms
+        c_vp, c_vp, c_vp,                    # aoff, boff, regs
+        c_vp, c_vp, c_vp, c_vp,              # a, b, u, v
+        c_int, c_dbl, c_vp, c_int,           # max_iter, stop_thr, order, vec_aligned
+        c_vp, c_vp, c_vp, c_vp,              # niter, status, err_last, err_first
+        c_vp, c_size, c_vp]),                # workspace, ws_bytes, stream
+    "potamd_sqeuclidean": (c_int, [
+        c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_i64, c_i64, c_vp, c_i64,
+        c_i64, c_int, c_int, c_int, c_vp]),
+}
+
+CONVERGED, MAX_ITER, NUMERICAL, NEED_EXACT = 0, 1, 2, 3
+F32, F64 = 0, 1
+LOG, SCALING = 0, 1
+COL_FIRST, ROW_FIRST = 0, 1
+
+_lock = threading.Lock()
+_lib = None
+
+
+class LibraryError(RuntimeError):
+    """The HIP library is missing, incomplete or returned an error."""
+
+
+def load():
+    """Load and bind the library once; raises LibraryError if unusable."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise LibraryError(
+                f"{LIB_PATH} not found: build it with `python -m pot_amd.build`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            try:
+                fn = getattr(lib, name)
+            except AttributeError as e:
+                raise LibraryError(f"{LIB_PATH} lacks symbol {name}") from e
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(code, what):
+    """Raise on negative return codes of the C ABI."""
+    if code < 0:
+        msg = load().potamd_status_string(code).decode()
+        raise LibraryError(f"{what} failed: {msg} (code {code})")
+    return code

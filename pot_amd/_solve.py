@@ -1,0 +1,123 @@
+"""Thin device-tensor wrappers over the C ABI (no argument semantics here)."""
+
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from . import _device, _lib
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+@dataclass
+class DenseResult:
+    u: torch.Tensor        # (B, n) float64: log potentials or scalings
+    v: torch.Tensor        # (B, m) float64
+    niter: int             # the reference's `ii` at the stop
+    errs: list             # marginal errors of the check iterations
+    status: int            # _lib.CONVERGED / MAX_ITER / NUMERICAL
+    bad_iter: int
+
+
+def dense_solve(M, a, b, u0, v0, reg, *, domain, order, kernel_given=False,
+                max_iter=1000, stop_thr=1e-9, err_norm=0, batch=None,
+                bstride=None, tile=None):
+    """Run potamd_sinkhorn_dense.
+
+    M: (n, m) or (B, n, m) device tensor (float32/float64), rows contiguous.
+    a: (B, n), b: (B, m), u0: (B, n), v0: (B, m) float64 device tensors.
+    ``batch``/``bstride`` override the batch layout (bstride = 0 shares M).
+    """
+    lib = _lib.load()
+    dev = M.device
+    code = _lib.F32 if M.dtype == torch.float32 else _lib.F64
+    if M.dim() == 2:
+        n, m = M.shape
+        ldm = M.stride(0)
+        B = batch or 1
+        bs = 0 if bstride is None else bstride
+    else:
+        B, n, m = M.shape
+        ldm = M.stride(1)
+        bs = M.stride(0) if bstride is None else bstride
+    if M.stride(-1) != 1:
+        raise ValueError("cost rows must be contiguous")
+    T, cpt, rpb = tile or (0, 0, 0)
+    ws_bytes = lib.potamd_dense_workspace_size(code, B, n, m, rpb)
+    if ws_bytes == 0:
+        raise _lib.LibraryError("invalid shape for the dense solver")
+    ws = _device.workspace(ws_bytes, dev)
+    u = u0.to(torch.float64).contiguous().clone()
+    v = v0.to(torch.float64).contiguous().clone()
+    err_cap = max(1, max_iter // 10 + 2)
+    err_hist = torch.zeros(err_cap, dtype=torch.float64, device=dev)
+    info = (ctypes.c_int * 4)()
+    hc = _device.host_ctrl()
+    rc = lib.potamd_sinkhorn_dense(
+        code, domain, order, int(bool(kernel_given)),
+        _ptr(M), ldm, bs, B, n, m, float(reg) if reg is not None else 1.0,
+        _ptr(a), _ptr(b), _ptr(u), _ptr(v),
+        int(max_iter), float(stop_thr), int(err_norm), _ptr(err_hist), err_cap,
+        T, cpt, rpb, _ptr(ws), ws_bytes, ctypes.c_void_p(hc.data_ptr()),
+        ctypes.c_void_p(_device.stream_ptr(dev)), info)
+    _lib.check(rc, "potamd_sinkhorn_dense")
+    if rc == _lib.NEED_EXACT:
+        raise _lib.LibraryError(
+            "a column mass fell below the fp32 range at iteration %d; "
+            "use float64 inputs" % info[0])
+    n_err = min(info[1], err_cap)
+    errs = err_hist[:n_err].cpu().tolist() if n_err else []
+    return DenseResult(u=u.view(B, n), v=v.view(B, m), niter=info[0], errs=errs,
+                       status=info[2], bad_iter=info[3])
+
+
+def plan(M, u, v, reg, *, domain, kernel_given=False, a=None, b=None,
+         want_plan=True, want_log=False, want_sums=True, batch=None, bstride=None,
+         out_dtype=None):
+    """Run potamd_plan_dense; returns (P, logP, sums) (None where not asked)."""
+    lib = _lib.load()
+    dev = M.device
+    code = _lib.F32 if M.dtype == torch.float32 else _lib.F64
+    if M.dim() == 2:
+        n, m = M.shape
+        ldm = M.stride(0)
+        B = batch or 1
+        bs = 0 if bstride is None else bstride
+    else:
+        B, n, m = M.shape
+        ldm = M.stride(1)
+        bs = M.stride(0) if bstride is None else bstride
+    P = torch.empty((B, n, m), dtype=M.dtype, device=dev) if want_plan else None
+    L = torch.empty((B, n, m), dtype=M.dtype, device=dev) if want_log else None
+    sums = torch.empty((B, 4), dtype=torch.float64, device=dev) if want_sums else None
+    ws_bytes = lib.potamd_plan_workspace_size(B, n)
+    ws = _device.workspace(ws_bytes, dev)
+    rc = lib.potamd_plan_dense(
+        code, domain, int(bool(kernel_given)), _ptr(M), ldm, bs, B, n, m,
+        float(reg) if reg is not None else 1.0,
+        _ptr(u.contiguous()), _ptr(v.contiguous()),
+        _ptr(a) if a is not None else None, _ptr(b) if b is not None else None,
+        _ptr(P), _ptr(L), n * m, _ptr(sums), _ptr(ws), ws_bytes,
+        ctypes.c_void_p(_device.stream_ptr(dev)))
+    _lib.check(rc, "potamd_plan_dense")
+    return P, L, sums
+
+
+def sqeuclidean(X, Y, *, squared=True, clamp=True, zero_diag=False):
+    """(B, n, d), (B, m, d) -> (B, n, m) via potamd_sqeuclidean."""
+    lib = _lib.load()
+    X = X.contiguous()
+    Y = Y.contiguous()
+    B, n, d = X.shape
+    m = Y.shape[1]
+    M = torch.empty((B, n, m), dtype=X.dtype, device=X.device)
+    code = _lib.F32 if X.dtype == torch.float32 else _lib.F64
+    rc = lib.potamd_sqeuclidean(
+        code, _ptr(X), _ptr(Y), B, n, m, d, n * d, m * d, _ptr(M), m, n * m,
+        int(squared), int(clamp), int(zero_diag),
+        ctypes.c_void_p(_device.stream_ptr(X.device)))
+    _lib.check(rc, "potamd_sqeuclidean")
+    return M

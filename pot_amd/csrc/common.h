@@ -1,0 +1,98 @@
+// Shared device helpers for the gfx950 Sinkhorn kernels.
+//
+// Conventions used by every kernel in this directory:
+//  * potentials live in HBM as float64 in base-2 log units (u2 = u / ln 2),
+//    so that the element exponent is an exp2, the native v_exp_f32;
+//  * the cost scale s2 = -log2(e) / reg is carried as an fp32 hi/lo pair so
+//    that M * s2 is exact to ~1e-14 relative before the single fp32 rounding;
+//  * wavefronts are 64 lanes; reductions use cross-lane shuffles inside a
+//    wave and LDS across waves.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace potamd {
+
+constexpr int kWave = 64;
+
+// Log / scaling domains of the solver.
+enum Domain : int { kLog = 0, kScaling = 1 };
+// Iteration order: column update first (ot.sinkhorn / sinkhorn_log /
+// sinkhorn_knopp) or row update first (ot.batch projections).
+enum Order : int { kColFirst = 0, kRowFirst = 1 };
+
+// Control block shared by the kernels of one solve (device memory).
+struct Ctrl {
+  int done;        // 1 once the solve has stopped; later kernels return early
+  int status;      // 0 converged, 1 max-iter, 2 numerical, 3 needs exact column pass
+  int niter;       // the reference's `ii` at the stop
+  int bad_iter;    // scaling domain: iteration whose update produced 0/nan/inf
+  unsigned counter;  // last-block-done ticket of the merge kernel (self-resetting)
+  int n_err;       // number of err values written
+  int need_exact;  // log domain: a column mass fell below the trusted range
+  int pad1;
+  double err;      // last marginal error
+};
+
+enum Status : int {
+  kConverged = 0,
+  kMaxIter = 1,
+  kNumerical = 2,
+  kNeedExact = 3,
+};
+
+__device__ __forceinline__ float exp2_fast(float x) {
+  return __builtin_amdgcn_exp2f(x);
+}
+__device__ __forceinline__ double exp2_fast(double x) { return exp2(x); }
+
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, kWave));
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, kWave));
+  return x;
+}
+
+// Workgroup barrier that orders LDS traffic only: waits for this wave's LDS
+// operations, then s_barrier.  Outstanding global loads stay in flight, so a
+// row prefetched before the barrier is not drained by it.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Element exponent in base 2: x2 = M * s2 + t with s2 = s_hi + s_lo.
+__device__ __forceinline__ float scaled_exponent(float m, float s_hi, float s_lo,
+                                                 float t) {
+  return __builtin_fmaf(m, s_lo, __builtin_fmaf(m, s_hi, t));
+}
+__device__ __forceinline__ double scaled_exponent(double m, double s_hi, double s_lo,
+                                                  double t) {
+  return fma(m, s_hi, t) + m * s_lo;
+}
+
+// Real type used for element arithmetic: fp32 for fp32 costs, fp64 for fp64.
+template <typename TM> struct Real { using type = float; };
+template <> struct Real<double> { using type = double; };
+
+__device__ __forceinline__ bool in_safe_range(double s) {
+  // 2^-60 .. 2^60: entries that underflowed are < 2^-66 of the sum, nothing
+  // overflowed (exp2 of fp32 saturates at 2^128).
+  return s >= 8.673617379884035e-19 && s <= 1.152921504606847e18;
+}
+
+}  // namespace potamd

@@ -1,0 +1,128 @@
+// Internal C++ interface between the kernel files and the C-ABI layer.
+#pragma once
+
+#include "common.h"
+
+namespace potamd {
+
+struct RowSweepArgs {
+  const void* M;      // B problems of n x ldm elements (TM)
+  int64_t ldm;        // row stride in elements
+  int64_t bstride;    // problem stride in elements
+  int n, m, rpb;      // rows, columns, rows per workgroup
+  float s_hi, s_lo;   // fp32 split of s2 (fp32 costs)
+  double s2;          // s2 in full (fp64 costs)
+  int kernel_given;   // scaling domain: M already holds K
+  int row_first;      // emit row-marginal error partials
+  int iter;           // iteration index (flag slot parity)
+  const double* a;    // B x n
+  const double* loga2;
+  const double* u_old;  // B x n
+  double* u_new;        // B x n
+  const double* v;      // B x m (current column potential / scaling)
+  void* colpart;        // B x G x m partial column sums (float or double)
+  double* rowerr;       // B x G partial row-marginal err^2
+  int* bad;             // [2] numerical-guard slots (scaling domain)
+  const Ctrl* ctrl;
+};
+
+struct MergeArgs {
+  int n, m, G, B;
+  int iter, max_iter;
+  int order, domain;
+  const double* b;      // B x m
+  const double* logb2;  // B x m
+  const double* v_cur;  // B x m
+  double* v_next;       // B x m
+  const void* colpart;  // B x G x m (float or double)
+  const double* rowerr; // B x G
+  double* errpart;      // B x gridDim.x
+  int* bad;             // [2]
+  Ctrl* ctrl;
+  double* err_hist;
+  int err_cap;
+  double stop_thr;
+  int prologue;         // scaling prologue: v1 = b / (K^T u0), no check
+  int err_norm;         // batch error: 0 max over problems, 1 Frobenius
+};
+
+struct ColLseArgs {
+  const void* M;
+  int64_t ldm, bstride;
+  int n, m, rpb;
+  double s2;
+  const double* u2;   // B x n
+  double* pmax;       // B x G x m
+  double* psum;
+};
+
+struct PlanArgs {
+  const void* M;
+  int64_t ldm, bstride;
+  int n, m, rpb;
+  int domain, kernel_given;
+  double reg;
+  const double* u;   // B x n natural-log potentials (log) or scalings
+  const double* v;   // B x m
+  const double* a;   // B x n (KL sums; may be null)
+  const double* b;   // B x m (KL sums; may be null)
+  void* P;           // plan out (dtype of M, row stride m), may be null
+  void* logP;        // log plan out (log domain), may be null
+  int64_t pstride;   // problem stride of P / logP
+  double* partial;   // B x blocks x 4
+};
+
+struct DistArgs {
+  const void* X;
+  const void* Y;
+  void* M;
+  int n, m, d;
+  int64_t xstride, ystride, mstride, ldm;
+  int squared, clamp, zero_diag;
+};
+
+struct BatchArgs {
+  const void* M;         // packed costs (float)
+  const int64_t* moff;   // [B] element offset of problem p's cost
+  const int* ns;         // [B] rows
+  const int* ms;         // [B] columns
+  const int* ldms;       // [B] row stride
+  const int64_t* aoff;   // [B] offset of problem p in a / u (length n_p)
+  const int64_t* boff;   // [B] offset of problem p in b / v (length m_p)
+  const double* regs;    // [B]
+  const double* a;
+  const double* b;
+  double* u;             // in: u0 (natural log), out: log_u
+  double* v;             // out: log_v
+  const int* order;      // [B] launch order (largest first) or null
+  int max_iter;
+  double stop_thr;
+  int* niter;            // [B]
+  int* status;           // [B]
+  double* err_last;      // [B] last checked error (may be null)
+  double* err_first;     // [B] error at the first check (may be null)
+  double* u2;            // workspace: base-2 row potentials
+  double* loga2;         // workspace
+  double* sbuf;          // workspace: per-row shifted sums
+};
+
+int batch_cpt(int max_m);
+hipError_t launch_batch_log(int cpt, int vec_ok, const BatchArgs& a, int B, hipStream_t s);
+
+int plan_blocks(int n);
+hipError_t launch_plan(int dtype, PlanArgs a, int B, double* sums, hipStream_t s);
+hipError_t launch_sqeuclid(int dtype, const DistArgs& a, int B, hipStream_t s);
+
+bool row_config_valid(int dtype, int T, int cpt);
+void row_config_default(int dtype, int m, int* T, int* cpt);
+hipError_t launch_row_sweep(int dtype, int vec_ok, int T, int cpt, int domain, int mode,
+                            const RowSweepArgs& a, int G, int B, hipStream_t s);
+hipError_t launch_col_merge(int dtype, const MergeArgs& a, int B, hipStream_t s);
+hipError_t launch_col_lse(int dtype, const ColLseArgs& a, int G, int B, const double* logb2,
+                          double* v_out, hipStream_t s);
+hipError_t launch_log2(int64_t count, const double* x, double* y, hipStream_t s);
+hipError_t launch_scale_copy(int64_t count, const double* x, double f, double* y, hipStream_t s);
+hipError_t launch_fill(int64_t count, double val, double* y, hipStream_t s);
+hipError_t launch_ctrl_init(Ctrl* c, int* bad, hipStream_t s);
+
+}  // namespace potamd

@@ -1,0 +1,97 @@
+"""CPU checks of the C-ABI library: it is built, current, exports every symbol
+include/pot_amd.h declares, and rejects bad arguments before touching a GPU."""
+
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from pot_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
+
+
+def _declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(potamd_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exists_and_is_current():
+    assert os.path.exists(_lib.LIB_PATH), "run python -m pot_amd.build"
+    lib_mtime = os.path.getmtime(_lib.LIB_PATH)
+    srcs = glob.glob(os.path.join(ROOT, "pot_amd", "csrc", "*")) + HEADERS
+    stale = [s for s in srcs if os.path.getmtime(s) > lib_mtime + 1e-6]
+    assert not stale, f"libpot_amd.so is older than {stale}; rebuild"
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    declared = _declared_functions()
+    assert declared, "no declarations found"
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
+    _lib.load()
+
+
+def test_status_strings_and_version():
+    lib = _lib.load()
+    assert lib.potamd_version() >= 100
+    for code in (0, 1, 2, 3, -1, -2, -3, -4):
+        assert lib.potamd_status_string(code)
+
+
+def test_dense_argument_validation():
+    lib = _lib.load()
+    P = ctypes.c_void_p(0x1000)
+    info = (ctypes.c_int * 4)()
+
+    def call(**kw):
+        args = dict(dtype=0, domain=0, order=0, kg=0, M=P, ldm=8, bs=0, B=1, n=8, m=8,
+                    reg=0.1, max_iter=10, err_norm=0, ws=None, wsb=0)
+        args.update(kw)
+        return lib.potamd_sinkhorn_dense(
+            args["dtype"], args["domain"], args["order"], args["kg"], args["M"],
+            args["ldm"], args["bs"], args["B"], args["n"], args["m"], args["reg"],
+            P, P, P, P, args["max_iter"], 1e-9, args["err_norm"], P, 4, 0, 0, 0,
+            args["ws"], args["wsb"], None, None, info)
+
+    assert call() == -4                      # valid, but no workspace
+    assert call(dtype=5) == -1
+    assert call(ldm=4) == -1                 # ldm < m
+    assert call(reg=0.0) == -1               # log domain needs reg > 0
+    assert call(domain=1, reg=0.0) == -4     # scaling domain accepts reg = 0 (nan guard)
+    assert call(max_iter=0) == -1
+    assert call(err_norm=3) == -1
+    assert call(m=1 << 20, ldm=1 << 20) == -3  # beyond the compiled tile set
+
+
+def test_plan_and_dist_argument_validation():
+    lib = _lib.load()
+    P = ctypes.c_void_p(0x1000)
+    rc = lib.potamd_plan_dense(0, 0, 0, P, 8, 0, 1, 8, 8, 0.1, P, P, None, None, P, None,
+                               64, None, None, 0, None)
+    assert rc == -4
+    rc = lib.potamd_plan_dense(0, 0, 0, P, 4, 0, 1, 8, 8, 0.1, P, P, None, None, P, None,
+                               64, None, None, 0, None)
+    assert rc == -1
+    rc = lib.potamd_sqeuclidean(0, P, P, 1, 4, 4, 0, 16, 16, P, 4, 16, 1, 1, 0, None)
+    assert rc == -1
+
+
+def test_no_gpu_means_loud_failure():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    import numpy as np
+    import pot_amd
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        pot_amd.sinkhorn([0.5, 0.5], [0.5, 0.5], np.eye(2), 1.0)

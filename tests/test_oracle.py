@@ -1,0 +1,278 @@
+"""Pins the float64 oracle (CPU only).
+
+Each check is independent of the oracle's own loop: closed forms, separable
+costs, an L-BFGS solve of the entropic dual, a pure-Python first iterate, and
+the invariants the reference tests hold (test/test_bregman.py,
+test/batch/test_solve_batch.py).
+"""
+
+import math
+import warnings
+
+import numpy as np
+import pytest
+from scipy.optimize import minimize
+
+import oracle.batch as ob
+import oracle.sinkhorn as osk
+import oracle.utils as ou
+
+
+def _problem(n, m, seed, reg_scale=1.0):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, 2))
+    y = rng.standard_normal((m, 2)) + 0.7
+    M = ((x[:, None] - y[None]) ** 2).sum(-1)
+    return rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(m)), M / M.max() * reg_scale
+
+
+# --- closed forms ----------------------------------------------------------
+
+def test_symmetric_2x2_doctest_values():
+    # ot/bregman/_sinkhorn.py:123-129, :342-347, :563-569, :766-772
+    a = [0.5, 0.5]
+    M = [[0.0, 1.0], [1.0, 0.0]]
+    exp = np.array([[0.36552929, 0.13447071], [0.13447071, 0.36552929]])
+    for method in ("sinkhorn", "sinkhorn_log"):
+        P = osk.sinkhorn(a, a, M, 1, method=method)
+        np.testing.assert_allclose(P, exp, atol=5e-9)
+        np.testing.assert_allclose(P[0, 0], math.e / (2 * (1 + math.e)), rtol=1e-12)
+        loss = osk.sinkhorn2(a, a, M, 1, method=method)
+        np.testing.assert_allclose(loss, 0.26894142136999516, rtol=1e-12)
+        np.testing.assert_allclose(loss, 1 / (1 + math.e), rtol=1e-12)
+
+
+def _general_2x2(a, b, M, reg):
+    """Entropic 2x2 plan from its one free entry: P11 P22 / (P12 P21) = kappa."""
+    kappa = math.exp(-(M[0][0] + M[1][1] - M[0][1] - M[1][0]) / reg)
+    a1, b1 = a[0], b[0]
+    # p (1 - a1 - b1 + p) = kappa (a1 - p)(b1 - p)
+    A = 1 - kappa
+    B = (1 - a1 - b1) + kappa * (a1 + b1)
+    C = -kappa * a1 * b1
+    if abs(A) < 1e-15:
+        p = -C / B
+    else:
+        disc = math.sqrt(B * B - 4 * A * C)
+        roots = [(-B + disc) / (2 * A), (-B - disc) / (2 * A)]
+        lo, hi = max(0.0, a1 + b1 - 1), min(a1, b1)
+        p = [r for r in roots if lo - 1e-15 <= r <= hi + 1e-15][0]
+    return np.array([[p, a1 - p], [b1 - p, 1 - a1 - b1 + p]])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_general_2x2_closed_form(seed):
+    rng = np.random.default_rng(seed)
+    a = rng.dirichlet([1, 1])
+    b = rng.dirichlet([1, 1])
+    M = rng.random((2, 2))
+    reg = 0.3
+    exact = _general_2x2(a, b, M, reg)
+    for method in ("sinkhorn", "sinkhorn_log"):
+        P = osk.sinkhorn(a, b, M, reg, method=method, stopThr=1e-15, numItermax=5000,
+                         warn=False)
+        np.testing.assert_allclose(P, exact, rtol=1e-9, atol=1e-14)
+
+
+def test_separable_cost_gives_product_plan():
+    rng = np.random.default_rng(3)
+    a, b = rng.dirichlet(np.ones(7)), rng.dirichlet(np.ones(5))
+    M = rng.random(7)[:, None] + rng.random(5)[None, :]
+    for method in ("sinkhorn", "sinkhorn_log"):
+        with pytest.warns(UserWarning):
+            P = osk.sinkhorn(a, b, M, 0.05, method=method, numItermax=1, stopThr=0)
+        np.testing.assert_allclose(P, np.outer(a, b), rtol=1e-10)
+
+
+def test_large_reg_tends_to_product_plan():
+    a, b, M = _problem(9, 11, 4)
+    P = osk.sinkhorn_log(a, b, M, 1e4, stopThr=1e-14)
+    np.testing.assert_allclose(P, np.outer(a, b), rtol=1e-3)
+
+
+# --- brute force: L-BFGS on the smooth entropic dual -----------------------
+
+def _dual_plan(a, b, M, reg):
+    n, m = M.shape
+
+    def negdual(z):
+        f, g = z[:n], z[n:]
+        P = np.exp((f[:, None] + g[None, :] - M) / reg - 1.0)
+        val = f @ a + g @ b - reg * P.sum()
+        grad = np.concatenate([a - P.sum(1), b - P.sum(0)])
+        return -val, -grad
+
+    r = minimize(negdual, np.zeros(n + m), jac=True, method="L-BFGS-B",
+                 options={"gtol": 1e-14, "ftol": 1e-16, "maxiter": 20000, "maxcor": 50})
+    f, g = r.x[:n], r.x[n:]
+    return np.exp((f[:, None] + g[None, :] - M) / reg - 1.0)
+
+
+@pytest.mark.parametrize("method", ["sinkhorn", "sinkhorn_log"])
+def test_matches_dual_bruteforce(method):
+    a, b, M = _problem(8, 6, 5)
+    reg = 0.2
+    P_dual = _dual_plan(a, b, M, reg)
+    P = osk.sinkhorn(a, b, M, reg, method=method, stopThr=1e-14, numItermax=20000,
+                     warn=False)
+    np.testing.assert_allclose(P, P_dual, rtol=1e-6, atol=1e-12)
+
+
+# --- invariants --------------------------------------------------------------
+
+def test_fixed_point_and_marginals():
+    a, b, M = _problem(40, 30, 6)
+    reg = 0.05
+    P, lg = osk.sinkhorn_log(a, b, M, reg, stopThr=1e-12, numItermax=20000, log=True)
+    Mr = -M / reg
+    lu, lv = lg["log_u"], lg["log_v"]
+    # u is the last update: exact row fixed point, exact row marginal
+    ref_u = np.log(a) - np.log(np.exp(Mr + lv[None, :]).sum(1))
+    np.testing.assert_allclose(lu, ref_u, atol=1e-12)
+    np.testing.assert_allclose(P.sum(1), a, rtol=1e-12)
+    assert np.linalg.norm(P.sum(0) - b) < 1e-12
+    # primal value <P,M> + reg sum P log P equals reg (<log_u, a> + <log_v, b>)
+    # at the fixed point (log P = -M/reg + u + v with marginals a, b)
+    primal = np.sum(P * M) + reg * np.sum(P * np.log(P))
+    np.testing.assert_allclose(primal, reg * (lu @ a + lv @ b), rtol=1e-9)
+
+
+def test_gauge_invariance_of_warmstart():
+    a, b, M = _problem(12, 15, 7)
+    P0 = osk.sinkhorn_log(a, b, M, 0.1, numItermax=37, stopThr=0, warn=False)
+    P1 = osk.sinkhorn_log(a, b, M, 0.1, numItermax=37, stopThr=0, warn=False,
+                          warmstart=(np.full(12, 3.25), np.full(15, -3.25)))
+    np.testing.assert_allclose(P0, P1, rtol=1e-11)
+    # shifting M by row/column constants leaves the plan unchanged
+    P2 = osk.sinkhorn_log(a, b, M + np.arange(12)[:, None] * 0.01 + 0.02 * np.arange(15),
+                          0.1, stopThr=1e-13, numItermax=20000)
+    P3 = osk.sinkhorn_log(a, b, M, 0.1, stopThr=1e-13, numItermax=20000)
+    np.testing.assert_allclose(P2, P3, rtol=1e-9)
+
+
+def test_knopp_equals_log():
+    # test_bregman.py:282-314
+    a, b, M = _problem(25, 20, 8)
+    P1 = osk.sinkhorn_knopp(a, b, M, 0.1, stopThr=1e-13, numItermax=20000)
+    P2 = osk.sinkhorn_log(a, b, M, 0.1, stopThr=1e-13, numItermax=20000)
+    np.testing.assert_allclose(P1, P2, rtol=1e-8, atol=1e-16)
+
+
+# --- iteration semantics ----------------------------------------------------
+
+def _first_iterate_pure_python(a, b, M, reg):
+    n, m = len(a), len(b)
+    v = []
+    for j in range(m):
+        v.append(math.log(b[j]) - math.log(sum(math.exp(-M[i][j] / reg) for i in range(n))))
+    u = []
+    for i in range(n):
+        u.append(math.log(a[i]) - math.log(sum(math.exp(-M[i][j] / reg + v[j])
+                                               for j in range(m))))
+    colsum = [sum(math.exp(-M[i][j] / reg + u[i] + v[j]) for i in range(n)) for j in range(m)]
+    err = math.sqrt(sum((colsum[j] - b[j]) ** 2 for j in range(m)))
+    return u, v, err
+
+
+def test_first_iterate_and_check_cadence():
+    a, b, M = _problem(6, 5, 9)
+    reg = 0.07
+    u1, v1, err0 = _first_iterate_pure_python(a, b, M, reg)
+    with pytest.warns(UserWarning, match="did not converge"):
+        P, lg = osk.sinkhorn_log(a, b, M, reg, numItermax=1, stopThr=0, log=True)
+    assert lg["niter"] == 0 and len(lg["err"]) == 1
+    np.testing.assert_allclose(lg["err"][0], err0, rtol=1e-12)
+    np.testing.assert_allclose(lg["log_u"], u1, atol=1e-13)
+    np.testing.assert_allclose(lg["log_v"], v1, atol=1e-13)
+    # 11 iterations: checks at ii = 0 and 10
+    with pytest.warns(UserWarning):
+        _, lg = osk.sinkhorn_log(a, b, M, reg, numItermax=11, stopThr=0, log=True)
+    assert lg["niter"] == 10 and len(lg["err"]) == 2
+    # a generous threshold stops at the first check, strict '<', no warning
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        _, lg = osk.sinkhorn_log(a, b, M, reg, stopThr=err0 * 1.0000001, log=True)
+    assert lg["niter"] == 0
+    with pytest.warns(UserWarning):
+        _, lg = osk.sinkhorn_log(a, b, M, reg, stopThr=err0, numItermax=5, log=True)
+    assert lg["niter"] == 4  # err0 < err0 is false
+
+
+def test_knopp_numerical_guard_and_warnings():
+    a = ou.make_1D_gauss(100, m=30, s=10)
+    b = ou.make_1D_gauss(100, m=40, s=10)
+    M = ou.dist0(100)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        with pytest.warns(UserWarning, match="numerical errors"):
+            osk.sinkhorn(a, b, M, 0, method="sinkhorn", warn=False)
+    for method in ("sinkhorn", "sinkhorn_log"):
+        with pytest.warns(UserWarning):
+            osk.sinkhorn(a, b, M, 1.0, method=method, stopThr=0, numItermax=1)
+        with warnings.catch_warnings():
+            warnings.simplefilter("error")
+            osk.sinkhorn2(a, b, M, 1, method=method, stopThr=0, numItermax=1, warn=False)
+    with pytest.raises(ValueError):
+        osk.sinkhorn(a, b, M, 1.0, method="new_method")
+
+
+def test_empty_weights_default_uniform():
+    _, _, M = _problem(10, 12, 10)
+    P0 = osk.sinkhorn_log([], [], M, 0.1)
+    P1 = osk.sinkhorn_log(np.ones(10) / 10, np.ones(12) / 12, M, 0.1)
+    np.testing.assert_allclose(P0, P1)
+
+
+def test_multi_histogram_log_equals_per_column():
+    rng = np.random.RandomState(0)
+    x = rng.randn(10, 2)
+    u = np.ones(10) / 10
+    b = rng.rand(10, 3)
+    b = b / b.sum(0, keepdims=True)
+    M = ou.dist(x, x)
+    for method in ("sinkhorn", "sinkhorn_log"):
+        loss0 = osk.sinkhorn(u, b, M, 0.1, method=method, stopThr=1e-10)
+        loss = [osk.sinkhorn2(u, b[:, k], M, 0.1, method=method, stopThr=1e-10)
+                for k in range(3)]
+        np.testing.assert_allclose(loss0, loss, atol=1e-4)
+
+
+# --- batch --------------------------------------------------------------------
+
+@pytest.mark.parametrize("solver", ["sinkhorn", "log_sinkhorn"])
+@pytest.mark.parametrize("reg_type", ["kl", "entropy"])
+def test_solve_batch_equals_per_problem_sinkhorn_log(solver, reg_type):
+    # test/batch/test_solve_batch.py:26-62 with the oracle's sinkhorn_log as ot.solve
+    rng = np.random.RandomState(0)
+    M = rng.rand(4, 16, 16)
+    res = ob.solve_batch(M, 0.1, max_iter=10000, tol=1e-5, solver=solver, reg_type=reg_type)
+    for i in range(4):
+        P = osk.sinkhorn_log(np.ones(16) / 16, np.ones(16) / 16, M[i], 0.1,
+                             numItermax=10000, stopThr=1e-5)
+        np.testing.assert_allclose(res["plan"][i], P, atol=1e-5)
+        np.testing.assert_allclose(res["value_linear"][i], np.sum(M[i] * P), atol=1e-4)
+
+
+def test_batch_log_and_plain_agree_and_losses():
+    rng = np.random.RandomState(0)
+    X = rng.rand(4, 4, 2)
+    M = ob.dist_batch(X, X)
+    r1 = ob.bregman_projection_batch(np.exp(-M / 0.01), max_iter=50, tol=1e-10)
+    r2 = ob.bregman_log_projection_batch(-M / 0.01, max_iter=50, tol=1e-10)
+    np.testing.assert_allclose(r1["T"], r2["T"], atol=1e-3)
+    res = ob.solve_batch(M, 0.1, max_iter=10)
+    np.testing.assert_allclose(res["value_linear"], ob.loss_linear_batch(M, res["plan"]),
+                               atol=1e-12)
+
+
+def test_dist_bruteforce():
+    # test/test_utils.py:207-227
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((13, 4))
+    y = rng.standard_normal((9, 4))
+    D = ou.dist(x, y)
+    brute = np.array([[np.sum((xi - yj) ** 2) for yj in y] for xi in x])
+    np.testing.assert_allclose(D, brute, atol=1e-14)
+    Db = ob.dist_batch(x[None], y[None])[0]
+    np.testing.assert_allclose(Db, brute, atol=1e-13)
+    np.testing.assert_allclose(ou.dist(x, y, "euclidean"), np.sqrt(brute), atol=1e-13)
