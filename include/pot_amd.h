@@ -119,9 +119,18 @@ int potamd_sqeuclidean(int dtype, const void* X, const void* Y, int batch, int n
                        int64_t xstride, int64_t ystride, void* M, int64_t ldm, int64_t mstride,
                        int squared, int clamp, int zero_diag, void* stream);
 
-/* Bytes of workspace needed by potamd_sinkhorn_log_batch (total_n = sum of
- * the problems' row counts). */
-size_t potamd_batch_workspace_size(int64_t total_n);
+/* Ragged batch of squared-Euclidean costs (float32), clamped at 0, with the
+ * per-problem maximum (the `M / M.max()` normalisation of the workloads).
+ * Problem p: points X[xoff[p] .. +ns[p]) and Y[yoff[p] .. +ms[p]) of
+ * dimension d (row-major), cost at M + moff[p] with row stride ldms[p];
+ * maxout[p] receives max_ij M (may be NULL).  Descriptor arrays live on the
+ * device; max_n / max_m bound the shapes.  Generalises ot.dist (ot/utils.py:
+ * 270-307) to many problems in one launch.
+ */
+int potamd_sqeuclidean_ragged(int batch, int max_n, int max_m, int d, const float* X,
+                              const float* Y, const int64_t* xoff, const int64_t* yoff,
+                              const int* ns, const int* ms, float* M, const int64_t* moff,
+                              const int* ldms, float* maxout, void* stream);
 
 /* Ragged batch of independent sinkhorn_log problems, one per workgroup.
  *
@@ -132,22 +141,55 @@ size_t potamd_batch_workspace_size(int64_t total_n);
  * entry point generalises it for many uneven problems).
  * M: packed float32 costs; problem p at M + moff[p], n_p x m_p with row stride
  *    ldms[p].  All descriptor arrays (moff, ns, ms, ldms, aoff, boff, regs,
- *    order) are device arrays of `batch` entries; max_m >= every m_p (<= 1024).
+ *    order) are device arrays of `batch` entries; max_n >= every n_p and
+ *    max_m >= every m_p (both <= 1024).
  * a, u: problem p's rows at offset aoff[p]; b, v: columns at boff[p].
  *    u holds u0 on entry (zeros = the reference's cold start).
- * order: launch order of the problems (e.g. largest first) or NULL.
- * vec_aligned: 1 if M is 16-byte aligned and every moff, ldms, m_p is a
- *    multiple of 4 (vector loads), else 0.
+ * order: launch order of the problems (e.g. most work first) or NULL.
+ * vec_aligned: 1 if M is 16-byte aligned and every moff, ldms is a multiple
+ *    of 4 (vector loads), else 0.
+ * fallback_thr: > 0 enables the warm-start fallback: a problem whose first
+ *    checked error exceeds it restarts from u0 = 0 (status gets bit 4).
  * niter, status: per-problem outputs (device int arrays); err_last / err_first:
  *    last and first checked marginal errors (device, may be NULL).
+ * waves: wavefronts per problem (4 or 8; 0 = default).
  */
-int potamd_sinkhorn_log_batch(int dtype, int batch, int max_m, const void* M,
+int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int max_m, const void* M,
                               const int64_t* moff, const int* ns, const int* ms, const int* ldms,
                               const int64_t* aoff, const int64_t* boff, const double* regs,
                               const double* a, const double* b, double* u, double* v,
-                              int max_iter, double stop_thr, const int* order, int vec_aligned,
-                              int* niter, int* status, double* err_last, double* err_first,
-                              void* workspace, size_t ws_bytes, void* stream);
+                              int max_iter, double stop_thr, double fallback_thr,
+                              const int* order, int vec_aligned, int* niter, int* status,
+                              double* err_last, double* err_first, int waves, void* stream);
+
+/* ---- learned warm start (no counterpart in the reference) ------------- */
+
+/* bf16 MFMA GEMM with fused epilogue, the linear layers of the warm-start
+ * network (forward Y = X W, backward dX = dY W^T, dW = X^T dY):
+ *   C[M x N] = alpha * op(A)[M x K] op(B)[K x N]  (+ bias[N])
+ *              (+ rowvec[i / seg_len][N]) (relu) (* (mask[i][j] > 0))
+ * op(A)[i][k] = transA ? A[k*lda + i] : A[i*lda + k]; op(B) likewise with
+ * op(B)[k][j] = transB ? B[j*ldb + k] : B[k*ldb + j].  A, B, mask bf16.
+ * out_kind: 0 bf16 store, 1 fp32 store, 2 fp32 atomic add (split-K over
+ * `splits` workgroups; bias / rowvec must then be NULL).
+ */
+int potamd_gemm_bf16(int M, int N, int K, const void* A, int64_t lda, int transA,
+                     const void* B, int64_t ldb, int transB, void* C, int64_t ldc,
+                     int out_kind, float alpha, const float* bias, const float* rowvec,
+                     int seg_len, int relu, const void* mask, int64_t ldmask, int splits,
+                     void* stream);
+
+/* Small kernels of the warm-start network's forward, backward and Adam step,
+ * selected by `op` (argument meaning per op in pot_amd/warmstart/net.py):
+ *  0 build inputs, 1 segment mean, 2 vector x matrix, 3 vector x matrix^T,
+ *  4 output head, 5 centred weighted loss + gradient, 6 head backward,
+ *  7 head weight gradient, 8 column sums, 9 segment sums, 10 pooling
+ *  backward, 11 outer-product accumulation, 12 Adam step, 13 fp32 -> bf16.
+ * hp: 7 floats of hyper-parameters (Adam) or NULL.
+ */
+int potamd_mlp_op(int op, int rows, int seg_len, int P, int k, int n, int64_t count, int flag,
+                  double scalar, const float* hp, const void* x, const void* y, const void* z,
+                  const void* w, void* out, void* out2, void* stream);
 
 #ifdef __cplusplus
 }

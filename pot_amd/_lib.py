@@ -34,15 +34,23 @@ SIGNATURES = {
     "potamd_plan_dense": (c_int, [
         c_int, c_int, c_int, c_vp, c_i64, c_i64, c_int, c_int, c_int, c_dbl,
         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_size, c_vp]),
-    "potamd_batch_workspace_size": (c_size, [c_i64]),
     "potamd_sinkhorn_log_batch": (c_int, [
-        c_int, c_int, c_int, c_vp,           # dtype, batch, max_m, M
+        c_int, c_int, c_int, c_int, c_vp,    # dtype, batch, max_n, max_m, M
         c_vp, c_vp, c_vp, c_vp,              # moff, ns, ms, ldms
         c_vp, c_vp, c_vp,                    # aoff, boff, regs
         c_vp, c_vp, c_vp, c_vp,              # a, b, u, v
-        c_int, c_dbl, c_vp, c_int,           # max_iter, stop_thr, order, vec_aligned
+        c_int, c_dbl, c_dbl, c_vp, c_int,    # max_iter, stop_thr, fallback_thr, order, vec
         c_vp, c_vp, c_vp, c_vp,              # niter, status, err_last, err_first
-        c_vp, c_size, c_vp]),                # workspace, ws_bytes, stream
+        c_int, c_vp]),                       # waves, stream
+    "potamd_sqeuclidean_ragged": (c_int, [
+        c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+        c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "potamd_gemm_bf16": (c_int, [
+        c_int, c_int, c_int, c_vp, c_i64, c_int, c_vp, c_i64, c_int, c_vp, c_i64,
+        c_int, ctypes.c_float, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_int, c_vp]),
+    "potamd_mlp_op": (c_int, [
+        c_int, c_int, c_int, c_int, c_int, c_int, c_i64, c_int, c_dbl, c_vp,
+        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "potamd_sqeuclidean": (c_int, [
         c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_i64, c_i64, c_vp, c_i64,
         c_i64, c_int, c_int, c_int, c_vp]),

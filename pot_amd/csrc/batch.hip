@@ -11,20 +11,25 @@
 // and each of its 64 lanes owns CPT columns (4 consecutive per 16-byte
 // load).  A row's log-sum-exp is a wave-local reduction (no barrier); the
 // column sums of the next half-step accumulate in registers and are merged
-// across the W waves through LDS once per sweep.  As in the dense solver,
-// the row exponent is shifted by the previous row potential, so one exp2
-// per element suffices and rows whose shifted sum leaves [2^-60, 2^60] are
-// redone with their maximum as the shift.
+// across the W waves through LDS once per sweep.  Everything a row needs
+// besides its cost (shift c_i, weight a_i, the column potentials) is in LDS,
+// stored lane-major so each LDS read is conflict-free.  As in the dense
+// solver, the row exponent is shifted by the previous row potential, so one
+// exp2 per element suffices; a row whose shifted sum leaves [2^-60, 2^60] is
+// redone with its maximum as the shift.
+//
+// Warm-start fallback: when fallback_thr > 0 and the first check of a
+// problem started from a given u0 exceeds it, the problem restarts from
+// u0 = 0 (the cold start) and reports status | 4.
 #include "internal.h"
 
 namespace potamd {
 
-constexpr int kBatchWaves = 4;
-constexpr int kBatchThreads = kBatchWaves * kWave;
+constexpr int kBatchMaxN = 1024;
 
-template <int CPT, int VEC>
-__global__ __launch_bounds__(kBatchThreads) void batch_log_kernel(BatchArgs A) {
-  constexpr int W = kBatchWaves;
+template <int CPT, int VEC, int W>
+__global__ __launch_bounds__(W * kWave) void batch_log_kernel(BatchArgs A) {
+  constexpr int T = W * kWave;
   constexpr int NCH = CPT / VEC;
   constexpr int MMAX = kWave * CPT;
   const int p = A.order ? A.order[blockIdx.x] : blockIdx.x;
@@ -35,27 +40,31 @@ __global__ __launch_bounds__(kBatchThreads) void batch_log_kernel(BatchArgs A) {
   const double* bp = A.b + A.boff[p];
   double* up = A.u + A.aoff[p];
   double* vp = A.v + A.boff[p];
-  double* u2 = A.u2 + A.aoff[p];
-  double* la2 = A.loga2 + A.aoff[p];
-  double* sbuf = A.sbuf + A.aoff[p];
   const double reg = A.regs[p];
   const double s2d = -1.4426950408889634 / reg;
   const float s_hi = (float)s2d;
   const float s_lo = (float)(s2d - (double)s_hi);
 
-  __shared__ double vsh[MMAX];          // column potentials, base 2
-  __shared__ float part[W][MMAX];       // per-wave column partials
+  // Lane-major slot of column j: lane (j / VEC) % 64, register q.
+  auto slot = [&](int j) {
+    const int q = (j / (VEC * kWave)) * VEC + (j % VEC);
+    return q * kWave + (j / VEC) % kWave;
+  };
+  __shared__ double vsh[MMAX];          // column potentials (base 2), lane-major
+  __shared__ float part[W][MMAX];       // per-wave column partials, lane-major
+  __shared__ double u2[kBatchMaxN];     // row potentials (base 2)
+  __shared__ double c2s[kBatchMaxN];    // row shifts log2 a_i - u2_i
+  __shared__ float af[kBatchMaxN];      // row weights
+  __shared__ float sh[kBatchMaxN];      // shifted row sums (fp32 mantissa)
   __shared__ double red[W];
   __shared__ int flag;
 
-  // Set-up: base-2 state, log2 a.
-  for (int i = tid; i < n; i += kBatchThreads) {
+  for (int i = tid; i < n; i += T) {
     u2[i] = up[i] * 1.4426950408889634;
-    la2[i] = log2(ap[i]);
+    af[i] = (float)ap[i];
   }
   __syncthreads();
 
-  auto col = [&](int q) { return ((q / VEC) * kWave + lane) * VEC + (q % VEC); };
   auto load_row = [&](int i, float* dst) {
     const float* row = Mp + (int64_t)i * ldm;
 #pragma unroll
@@ -65,10 +74,11 @@ __global__ __launch_bounds__(kBatchThreads) void batch_log_kernel(BatchArgs A) {
         if constexpr (VEC == 4) {
           typedef float f4 __attribute__((ext_vector_type(4)));
           f4 x = __builtin_nontemporal_load(reinterpret_cast<const f4*>(row + j0));
+          // Columns past m (row padding) may hold anything: zero them.
           dst[k * 4 + 0] = x.x;
-          dst[k * 4 + 1] = x.y;
-          dst[k * 4 + 2] = x.z;
-          dst[k * 4 + 3] = x.w;
+          dst[k * 4 + 1] = j0 + 1 < m ? x.y : 0.f;
+          dst[k * 4 + 2] = j0 + 2 < m ? x.z : 0.f;
+          dst[k * 4 + 3] = j0 + 3 < m ? x.w : 0.f;
         } else {
           dst[k] = __builtin_nontemporal_load(row + j0);
         }
@@ -79,8 +89,12 @@ __global__ __launch_bounds__(kBatchThreads) void batch_log_kernel(BatchArgs A) {
     }
   };
 
-  // Prologue: v_1 = log b - LSE_i(Mr + u_0), online (max, sum) per column
-  // over this wave's rows, merged across waves as per-wave LSE values.
+  int status = kMaxIter, niter = A.max_iter - 1, restarted = 0;
+  double err_last = -1.0, err_first = -1.0;
+  const bool warm = A.fallback_thr > 0.0;
+
+restart:
+  // ---- prologue: v_1 = log b - LSE_i(Mr + u_0) per column ----
   {
     float mx[CPT], sm[CPT];
 #pragma unroll
@@ -97,34 +111,32 @@ __global__ __launch_bounds__(kBatchThreads) void batch_log_kernel(BatchArgs A) {
         const float x = scaled_exponent(cur[q], s_hi, s_lo, ui);
         const float d = x - mx[q];
         const float e = exp2_fast(-fabsf(d));
-        if (d > 0.f) {
-          sm[q] = __builtin_fmaf(sm[q], e, 1.f);
-          mx[q] = x;
-        } else {
-          sm[q] += e;
-        }
+        const bool up_ = d > 0.f;
+        sm[q] = up_ ? __builtin_fmaf(sm[q], e, 1.f) : sm[q] + e;
+        mx[q] = up_ ? x : mx[q];
       }
     }
 #pragma unroll
     for (int q = 0; q < CPT; ++q)
-      part[wid][col(q)] = (sm[q] > 0.f) ? mx[q] + __builtin_amdgcn_logf(sm[q]) : -__builtin_inff();
+      part[wid][q * kWave + lane] =
+          (sm[q] > 0.f) ? mx[q] + __builtin_amdgcn_logf(sm[q]) : -__builtin_inff();
     __syncthreads();
-    for (int j = tid; j < m; j += kBatchThreads) {
+    for (int j = tid; j < m; j += T) {
+      const int sj = slot(j);
       float g = -__builtin_inff();
 #pragma unroll
-      for (int w = 0; w < W; ++w) g = fmaxf(g, part[w][j]);
+      for (int w = 0; w < W; ++w) g = fmaxf(g, part[w][sj]);
       double s = 0.0;
 #pragma unroll
-      for (int w = 0; w < W; ++w) s += exp2((double)part[w][j] - (double)g);
+      for (int w = 0; w < W; ++w) s += exp2((double)part[w][sj] - (double)g);
       const double bj = bp[j];
-      vsh[j] = bj > 0.0 ? log2(bj) - ((double)g + log2(s)) : -__builtin_inf();
+      vsh[sj] = bj > 0.0 ? log2(bj) - ((double)g + log2(s)) : -__builtin_inf();
     }
-    for (int j = m + tid; j < MMAX; j += kBatchThreads) vsh[j] = -__builtin_inf();
+    for (int j = m + tid; j < MMAX; j += T) vsh[slot(j)] = -__builtin_inf();
+    for (int i = tid; i < n; i += T) c2s[i] = log2(ap[i]) - u2[i];
     __syncthreads();
   }
 
-  int status = kMaxIter, niter = A.max_iter - 1;
-  double err_last = -1.0, err_first = -1.0;
   float cur[CPT], nxt[CPT];
   for (int ii = 0; ii < A.max_iter; ++ii) {
     // ---- row sweep: u from v, column sums of P(u_new, v) ----
@@ -135,66 +147,70 @@ __global__ __launch_bounds__(kBatchThreads) void batch_log_kernel(BatchArgs A) {
     if (i < n) load_row(i, cur);
     for (; i < n; i += W) {
       if (i + W < n) load_row(i + W, nxt);
-      const double a_i = ap[i];
-      if (a_i > 0.0) {
-        double c2 = la2[i] - u2[i];
+      const float a_i = af[i];
+      if (a_i > 0.f) {
+        double c2 = c2s[i];
         float s = 0.f;
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
-          const float t = (float)(vsh[col(q)] - c2);
+          const float t = (float)(vsh[q * kWave + lane] - c2);
           cur[q] = exp2_fast(scaled_exponent(cur[q], s_hi, s_lo, t));
           s += cur[q];
         }
-        double S = (double)wave_sum(s);
-        double seff = S;
-        if (!in_safe_range(S)) {
+        float S = wave_sum(s);
+        if (!in_safe_range((double)S)) {
           load_row(i, cur);
           float mxr = -__builtin_inff();
 #pragma unroll
           for (int q = 0; q < CPT; ++q) {
-            const float x = scaled_exponent(cur[q], s_hi, s_lo, (float)(vsh[col(q)] - c2));
+            const float x =
+                scaled_exponent(cur[q], s_hi, s_lo, (float)(vsh[q * kWave + lane] - c2));
             mxr = fmaxf(mxr, x);
           }
-          const double gmx = (double)wave_max(mxr);
-          if (__builtin_isfinite(gmx)) c2 += gmx;
+          const float gmx = wave_max(mxr);
+          if (__builtin_isfinite(gmx)) c2 += (double)gmx;
           s = 0.f;
 #pragma unroll
           for (int q = 0; q < CPT; ++q) {
-            const float t = (float)(vsh[col(q)] - c2);
+            const float t = (float)(vsh[q * kWave + lane] - c2);
             cur[q] = exp2_fast(scaled_exponent(cur[q], s_hi, s_lo, t));
             s += cur[q];
           }
-          S = (double)wave_sum(s);
-          seff = S * exp2(__builtin_isfinite(gmx) ? gmx : 0.0);
+          S = wave_sum(s);
+          if (lane == 0) c2s[i] = c2;  // the shift actually used
         }
-        if (lane == 0) sbuf[i] = seff;
-        const float kf = (float)a_i * __builtin_amdgcn_rcpf((float)S);
+        if (lane == 0) sh[i] = S;
+        const float kf = a_i * __builtin_amdgcn_rcpf(S);
 #pragma unroll
         for (int q = 0; q < CPT; ++q) acc[q] = __builtin_fmaf(cur[q], kf, acc[q]);
-      } else if (lane == 0) {
-        sbuf[i] = 0.0;
       }
 #pragma unroll
       for (int q = 0; q < CPT; ++q) cur[q] = nxt[q];
     }
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) part[wid][col(q)] = acc[q];
+    for (int q = 0; q < CPT; ++q) part[wid][q * kWave + lane] = acc[q];
     __syncthreads();
 
     // ---- row potentials, column sums, marginal error ----
-    for (int r = tid; r < n; r += kBatchThreads) {
-      const double S = sbuf[r];
-      u2[r] = ap[r] > 0.0 ? u2[r] - log2(S) : -__builtin_inf();
+    // u2_new = log2 a - c2_used - log2 S ; next shift c2 = log2 a - u2_new = c2_used + log2 S.
+    for (int r = tid; r < n; r += T) {
+      if (af[r] > 0.f) {
+        const double c = c2s[r] + log2((double)sh[r]);
+        c2s[r] = c;
+        u2[r] = log2(ap[r]) - c;
+      } else {
+        u2[r] = -__builtin_inf();
+      }
     }
     double e2 = 0.0;
-    const bool check = (ii % 10 == 0);
-    for (int j = tid; j < m; j += kBatchThreads) {
+    for (int j = tid; j < m; j += T) {
+      const int sj = slot(j);
       float cs = 0.f;
 #pragma unroll
-      for (int w = 0; w < W; ++w) cs += part[w][j];
+      for (int w = 0; w < W; ++w) cs += part[w][sj];
       const double d = (double)cs - bp[j];
       e2 += d * d;
-      part[0][j] = cs;
+      part[0][sj] = cs;
     }
     e2 = wave_sum(e2);
     if (lane == 0) red[wid] = e2;
@@ -205,35 +221,46 @@ __global__ __launch_bounds__(kBatchThreads) void batch_log_kernel(BatchArgs A) {
       for (int w = 0; w < W; ++w) t += red[w];
       const double err = sqrt(t);
       int stop = 0;
-      if (check) {
+      if (ii % 10 == 0) {
         err_last = err;
-        if (ii == 0) err_first = err;
-        if (err < A.stop_thr) stop = 1;
+        if (ii == 0) {
+          err_first = err;
+          if (warm && !restarted && err > A.fallback_thr) stop = 2;
+        }
+        if (stop == 0 && err < A.stop_thr) stop = 1;
       }
       flag = stop;
     }
     __syncthreads();
-    if (flag) {
+    if (flag == 2) {
+      // Warm start rejected: restart from the cold start u0 = 0.
+      restarted = 1;
+      for (int r = tid; r < n; r += T) u2[r] = 0.0;
+      __syncthreads();
+      goto restart;
+    }
+    if (flag == 1) {
       status = kConverged;
       niter = ii;
       break;
     }
     if (ii == A.max_iter - 1) break;
     // ---- column update v_j += log2 b_j - log2 colsum_j ----
-    for (int j = tid; j < m; j += kBatchThreads) {
+    for (int j = tid; j < m; j += T) {
+      const int sj = slot(j);
       const double bj = bp[j];
-      const double cs = (double)part[0][j];
-      vsh[j] = bj > 0.0 ? vsh[j] + log2(bj) - log2(cs) : -__builtin_inf();
+      const double cs = (double)part[0][sj];
+      vsh[sj] = bj > 0.0 ? vsh[sj] + log2(bj) - log2(cs) : -__builtin_inf();
     }
     __syncthreads();
   }
 
   // ---- outputs (natural-log potentials) ----
-  for (int i = tid; i < n; i += kBatchThreads) up[i] = u2[i] * 0.6931471805599453;
-  for (int j = tid; j < m; j += kBatchThreads) vp[j] = vsh[j] * 0.6931471805599453;
+  for (int i = tid; i < n; i += T) up[i] = u2[i] * 0.6931471805599453;
+  for (int j = tid; j < m; j += T) vp[j] = vsh[slot(j)] * 0.6931471805599453;
   if (tid == 0) {
     A.niter[p] = niter;
-    A.status[p] = status;
+    A.status[p] = status | (restarted ? 4 : 0);
     if (A.err_last) A.err_last[p] = err_last;
     if (A.err_first) A.err_first[p] = err_first;
   }
@@ -246,19 +273,25 @@ int batch_cpt(int max_m) {
   return 0;
 }
 
-hipError_t launch_batch_log(int cpt, int vec_ok, const BatchArgs& a, int B, hipStream_t s) {
-  dim3 grid(B), block(kBatchThreads);
-#define POTAMD_BATCH_CASE(C)                                                      \
-  if (cpt == C) {                                                                 \
-    if (vec_ok)                                                                   \
-      hipLaunchKernelGGL((batch_log_kernel<C, 4>), grid, block, 0, s, a);         \
-    else                                                                          \
-      hipLaunchKernelGGL((batch_log_kernel<C, 1>), grid, block, 0, s, a);         \
-    return hipGetLastError();                                                     \
+int batch_waves_default() { return 4; }
+
+hipError_t launch_batch_log(int cpt, int vec_ok, int waves, const BatchArgs& a, int B,
+                            hipStream_t s) {
+  dim3 grid(B);
+#define POTAMD_BATCH_CASE(C, WV)                                                      \
+  if (cpt == C && waves == WV) {                                                     \
+    if (vec_ok)                                                                      \
+      hipLaunchKernelGGL((batch_log_kernel<C, 4, WV>), grid, dim3(WV * kWave), 0, s, a); \
+    else                                                                             \
+      hipLaunchKernelGGL((batch_log_kernel<C, 1, WV>), grid, dim3(WV * kWave), 0, s, a); \
+    return hipGetLastError();                                                        \
   }
-  POTAMD_BATCH_CASE(4)
-  POTAMD_BATCH_CASE(8)
-  POTAMD_BATCH_CASE(16)
+  POTAMD_BATCH_CASE(4, 4)
+  POTAMD_BATCH_CASE(8, 4)
+  POTAMD_BATCH_CASE(16, 4)
+  POTAMD_BATCH_CASE(4, 8)
+  POTAMD_BATCH_CASE(8, 8)
+  POTAMD_BATCH_CASE(16, 8)
 #undef POTAMD_BATCH_CASE
   return hipErrorInvalidConfiguration;
 }

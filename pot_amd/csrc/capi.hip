@@ -397,29 +397,21 @@ extern "C" int potamd_sqeuclidean(int dtype, const void* X, const void* Y, int b
   return 0;
 }
 
-extern "C" size_t potamd_batch_workspace_size(int64_t total_n) {
-  if (total_n < 1) return 0;
-  return 3 * align_up((size_t)total_n * sizeof(double));
-}
-
-extern "C" int potamd_sinkhorn_log_batch(int dtype, int batch, int max_m, const void* M,
-                                         const int64_t* moff, const int* ns, const int* ms,
-                                         const int* ldms, const int64_t* aoff,
+extern "C" int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int max_m,
+                                         const void* M, const int64_t* moff, const int* ns,
+                                         const int* ms, const int* ldms, const int64_t* aoff,
                                          const int64_t* boff, const double* regs,
                                          const double* a, const double* b, double* u,
                                          double* v, int max_iter, double stop_thr,
-                                         const int* order, int vec_aligned, int* niter,
-                                         int* status, double* err_last, double* err_first,
-                                         void* workspace, size_t ws_bytes, void* stream) {
-  if (batch < 1 || max_m < 1 || !M || !moff || !ns || !ms || !ldms || !aoff || !boff || !regs ||
-      !a || !b || !u || !v || max_iter < 1 || !niter || !status)
+                                         double fallback_thr, const int* order, int vec_aligned,
+                                         int* niter, int* status, double* err_last,
+                                         double* err_first, int waves, void* stream) {
+  if (batch < 1 || max_n < 1 || max_m < 1 || !M || !moff || !ns || !ms || !ldms || !aoff ||
+      !boff || !regs || !a || !b || !u || !v || max_iter < 1 || !niter || !status)
     return POTAMD_EINVAL;
   if (dtype != POTAMD_F32) return POTAMD_EUNSUPPORTED;
   const int cpt = batch_cpt(max_m);
-  if (cpt == 0) return POTAMD_EUNSUPPORTED;
-  if (!workspace || ws_bytes < 3 * sizeof(double)) return POTAMD_EWORKSPACE;
-  const size_t third = ws_bytes / 3 / kAlign * kAlign;
-  if (third == 0) return POTAMD_EWORKSPACE;
+  if (cpt == 0 || max_n > 1024) return POTAMD_EUNSUPPORTED;
   BatchArgs ba;
   memset(&ba, 0, sizeof(ba));
   ba.M = M;
@@ -437,15 +429,103 @@ extern "C" int potamd_sinkhorn_log_batch(int dtype, int batch, int max_m, const 
   ba.order = order;
   ba.max_iter = max_iter;
   ba.stop_thr = stop_thr;
+  ba.fallback_thr = fallback_thr;
   ba.niter = niter;
   ba.status = status;
   ba.err_last = err_last;
   ba.err_first = err_first;
-  char* w = static_cast<char*>(workspace);
-  ba.u2 = reinterpret_cast<double*>(w);
-  ba.loga2 = reinterpret_cast<double*>(w + third);
-  ba.sbuf = reinterpret_cast<double*>(w + 2 * third);
   const int vec_ok = vec_aligned && ((uintptr_t)M % 16 == 0);
-  HIP_TRY(launch_batch_log(cpt, vec_ok, ba, batch, static_cast<hipStream_t>(stream)));
+  if (waves == 0) waves = batch_waves_default();
+  if (waves != 4 && waves != 8) return POTAMD_EINVAL;
+  HIP_TRY(launch_batch_log(cpt, vec_ok, waves, ba, batch, static_cast<hipStream_t>(stream)));
+  return 0;
+}
+
+extern "C" int potamd_sqeuclidean_ragged(int batch, int max_n, int max_m, int d, const float* X,
+                                         const float* Y, const int64_t* xoff,
+                                         const int64_t* yoff, const int* ns, const int* ms,
+                                         float* M, const int64_t* moff, const int* ldms,
+                                         float* maxout, void* stream) {
+  if (batch < 1 || max_n < 1 || max_m < 1 || d < 1 || !X || !Y || !xoff || !yoff || !ns || !ms ||
+      !M || !moff || !ldms)
+    return POTAMD_EINVAL;
+  RaggedDistArgs ra;
+  ra.X = X;
+  ra.Y = Y;
+  ra.xoff = xoff;
+  ra.yoff = yoff;
+  ra.ns = ns;
+  ra.ms = ms;
+  ra.d = d;
+  ra.M = M;
+  ra.moff = moff;
+  ra.ldms = ldms;
+  ra.maxout = maxout;
+  HIP_TRY(launch_sqeuclid_ragged(ra, batch, max_n, max_m, static_cast<hipStream_t>(stream)));
+  return 0;
+}
+
+extern "C" int potamd_gemm_bf16(int M, int N, int K, const void* A, int64_t lda, int transA,
+                                const void* B, int64_t ldb, int transB, void* C, int64_t ldc,
+                                int out_kind, float alpha, const float* bias, const float* rowvec,
+                                int seg_len, int relu, const void* mask, int64_t ldmask,
+                                int splits, void* stream) {
+  if (M < 1 || N < 1 || K < 1 || !A || !B || !C || out_kind < 0 || out_kind > 2 || splits < 1 ||
+      (rowvec && seg_len < 1) || (splits > 1 && (out_kind != 2 || bias || rowvec)))
+    return POTAMD_EINVAL;
+  if ((!transA && lda < K) || (transA && lda < M) || (!transB && ldb < N) || (transB && ldb < K) ||
+      ldc < N || (mask && ldmask < N))
+    return POTAMD_EINVAL;
+  GemmArgs g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.lda = lda;
+  g.transA = transA;
+  g.B = B;
+  g.ldb = ldb;
+  g.transB = transB;
+  g.C = C;
+  g.ldc = ldc;
+  g.out_kind = out_kind;
+  g.alpha = alpha;
+  g.bias = bias;
+  g.rowvec = rowvec;
+  g.seg_len = seg_len > 0 ? seg_len : 1;
+  g.relu = relu;
+  g.mask = mask;
+  g.ldmask = ldmask;
+  HIP_TRY(launch_gemm_bf16(g, splits, static_cast<hipStream_t>(stream)));
+  return 0;
+}
+
+extern "C" int potamd_mlp_op(int op, int rows, int seg_len, int P, int k, int n, int64_t count,
+                             int flag, double scalar, const float* hp, const void* x,
+                             const void* y, const void* z, const void* w, void* out, void* out2,
+                             void* stream) {
+  if (op < 0 || op > kOpToBf16 || !out) return POTAMD_EINVAL;
+  if ((op == kOpColSum || op == kOpHeadWGrad) && (k < 1 || k > 1024)) return POTAMD_EINVAL;
+  if ((op == kOpSegmentMean || op == kOpSegmentSum || op == kOpPoolBack || op == kOpLoss ||
+       op == kOpBuildInputs) && seg_len < 1)
+    return POTAMD_EINVAL;
+  MlpOp o;
+  o.op = op;
+  o.rows = rows;
+  o.seg_len = seg_len;
+  o.P = P;
+  o.k = k;
+  o.n = n;
+  o.count = count;
+  o.flag = flag;
+  o.scalar = scalar;
+  for (int q = 0; q < 7; ++q) o.hp[q] = hp ? hp[q] : 0.f;
+  o.x = x;
+  o.y = y;
+  o.z = z;
+  o.w = w;
+  o.out = out;
+  o.out2 = out2;
+  HIP_TRY(launch_mlp_op(o, static_cast<hipStream_t>(stream)));
   return 0;
 }

@@ -84,8 +84,8 @@ struct DistArgs {
 struct BatchArgs {
   const void* M;         // packed costs (float)
   const int64_t* moff;   // [B] element offset of problem p's cost
-  const int* ns;         // [B] rows
-  const int* ms;         // [B] columns
+  const int* ns;         // [B] rows (<= 1024)
+  const int* ms;         // [B] columns (<= 1024)
   const int* ldms;       // [B] row stride
   const int64_t* aoff;   // [B] offset of problem p in a / u (length n_p)
   const int64_t* boff;   // [B] offset of problem p in b / v (length m_p)
@@ -97,17 +97,88 @@ struct BatchArgs {
   const int* order;      // [B] launch order (largest first) or null
   int max_iter;
   double stop_thr;
+  double fallback_thr;   // > 0: restart cold when the first check exceeds it
   int* niter;            // [B]
   int* status;           // [B]
   double* err_last;      // [B] last checked error (may be null)
   double* err_first;     // [B] error at the first check (may be null)
-  double* u2;            // workspace: base-2 row potentials
-  double* loga2;         // workspace
-  double* sbuf;          // workspace: per-row shifted sums
 };
 
 int batch_cpt(int max_m);
-hipError_t launch_batch_log(int cpt, int vec_ok, const BatchArgs& a, int B, hipStream_t s);
+int batch_waves_default();
+hipError_t launch_batch_log(int cpt, int vec_ok, int waves, const BatchArgs& a, int B,
+                            hipStream_t s);
+
+struct RaggedDistArgs {
+  const float* X;        // packed points, problem p rows at xoff[p]
+  const float* Y;
+  const int64_t* xoff;   // [B] point offsets (rows)
+  const int64_t* yoff;
+  const int* ns;
+  const int* ms;
+  int d;
+  float* M;
+  const int64_t* moff;
+  const int* ldms;
+  float* maxout;         // [B] per-problem maximum cost (may be null)
+};
+
+hipError_t launch_sqeuclid_ragged(const RaggedDistArgs& a, int B, int max_n, int max_m,
+                                  hipStream_t s);
+
+struct GemmArgs {
+  int M, N, K;
+  const void* A;   // bf16
+  int64_t lda;
+  int transA;      // op(A)[i][k] = transA ? A[k*lda + i] : A[i*lda + k]
+  const void* B;   // bf16
+  int64_t ldb;
+  int transB;      // op(B)[k][j] = transB ? B[j*ldb + k] : B[k*ldb + j]
+  void* C;
+  int64_t ldc;
+  int out_kind;    // 0 bf16 store, 1 fp32 store, 2 fp32 atomic add
+  float alpha;
+  const float* bias;    // [N] or null
+  const float* rowvec;  // [M / seg_len][N] or null: row i adds rowvec[i / seg_len]
+  int seg_len;
+  int relu;
+  const void* mask;     // bf16 [M][ldmask] or null: zero where mask <= 0
+  int64_t ldmask;
+};
+
+enum MlpOpCode : int {
+  kOpBuildInputs = 0,
+  kOpSegmentMean = 1,
+  kOpVecMat = 2,
+  kOpVecMatT = 3,
+  kOpHead = 4,
+  kOpLoss = 5,
+  kOpHeadBack = 6,
+  kOpHeadWGrad = 7,
+  kOpColSum = 8,
+  kOpSegmentSum = 9,
+  kOpPoolBack = 10,
+  kOpOuterAcc = 11,
+  kOpAdam = 12,
+  kOpToBf16 = 13,
+};
+
+struct MlpOp {
+  int op, rows, seg_len, P, k, n;
+  int64_t count;
+  int flag;
+  double scalar;
+  float hp[7];
+  const void* x;
+  const void* y;
+  const void* z;
+  const void* w;
+  void* out;
+  void* out2;
+};
+
+hipError_t launch_gemm_bf16(const GemmArgs& g, int splits, hipStream_t s);
+hipError_t launch_mlp_op(const MlpOp& o, hipStream_t s);
 
 int plan_blocks(int n);
 hipError_t launch_plan(int dtype, PlanArgs a, int B, double* sums, hipStream_t s);

@@ -1,0 +1,207 @@
+"""Many uneven ``sinkhorn_log`` problems in one launch.
+
+The reference batch (ot/batch/_utils.py:165-290) needs one shape, one ``reg``
+and stops all problems together; this module generalises it: every problem
+keeps its own n, m, reg, initial ``u`` and stop, and each one follows
+``ot.bregman.sinkhorn_log`` exactly (ot/bregman/_sinkhorn.py:880-906), so a
+problem's ``niter`` and final potentials are those of a lone
+``sinkhorn_log`` call on it.
+
+Costs are packed row-major into one float32 buffer (row stride rounded up to
+4 for 16-byte loads).  A cost normalised by its maximum, ``M / max M``, is
+kept unnormalised in memory and the maximum is folded into ``reg``
+(``exp(-(M/max)/reg) = exp(-M/(reg*max))``).
+"""
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _device, _lib
+
+MAX_M = 1024
+MAX_N = 1024
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _round4(x):
+    return (x + 3) // 4 * 4
+
+
+@dataclass
+class RaggedProblems:
+    """Packed device representation of B problems."""
+    device: torch.device
+    ns: np.ndarray          # (B,) int
+    ms: np.ndarray
+    M: torch.Tensor         # packed float32 costs
+    moff: torch.Tensor      # (B,) int64
+    ns_d: torch.Tensor      # (B,) int32
+    ms_d: torch.Tensor
+    ldms_d: torch.Tensor
+    aoff: torch.Tensor      # (B,) int64 offsets into row arrays
+    boff: torch.Tensor      # (B,) int64 offsets into column arrays
+    a: torch.Tensor         # (sum n,) float64
+    b: torch.Tensor         # (sum m,) float64
+    regs: torch.Tensor      # (B,) float64 effective reg (max folded in)
+    cmax: torch.Tensor      # (B,) float32 cost maxima (1 when not normalised)
+
+    @property
+    def batch(self):
+        return len(self.ns)
+
+    def cost(self, p):
+        """Problem p's cost as stored (unnormalised), an (n, m) view."""
+        n, m = int(self.ns[p]), int(self.ms[p])
+        ld = _round4(m)
+        o = int(self.moff[p].item())
+        return self.M[o:o + n * ld].view(n, ld)[:, :m]
+
+
+def _layout(ns, ms, device):
+    ns = np.asarray(ns, dtype=np.int64)
+    ms = np.asarray(ms, dtype=np.int64)
+    if ms.max() > MAX_M or ns.max() > MAX_N:
+        raise ValueError(f"ragged solver supports n <= {MAX_N}, m <= {MAX_M}")
+    lds = (ms + 3) // 4 * 4
+    sizes = ns * lds
+    moff = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    aoff = np.concatenate([[0], np.cumsum(ns)[:-1]])
+    boff = np.concatenate([[0], np.cumsum(ms)[:-1]])
+    t = lambda x, dt: torch.tensor(x, dtype=dt, device=device)
+    return (ns, ms, int(sizes.sum()), t(moff, torch.int64), t(ns, torch.int32),
+            t(ms, torch.int32), t(lds, torch.int32), t(aoff, torch.int64),
+            t(boff, torch.int64))
+
+
+def _weights(ws, counts, device):
+    if ws is None:
+        return torch.cat([torch.full((int(c),), 1.0 / int(c), dtype=torch.float64)
+                          for c in counts]).to(device)
+    return torch.cat([torch.as_tensor(np.asarray(w), dtype=torch.float64).reshape(-1)
+                      for w in ws]).to(device)
+
+
+def _pack_points(xs, device):
+    """(sum n, d) float32 device tensor from a list of clouds or a (B, n, d) tensor."""
+    if isinstance(xs, torch.Tensor):
+        return xs.to(device=device, dtype=torch.float32).reshape(-1, xs.shape[-1]).contiguous()
+    return torch.cat([x.to(device=device, dtype=torch.float32) if isinstance(x, torch.Tensor)
+                      else torch.as_tensor(np.asarray(x, dtype=np.float32)).to(device)
+                      for x in xs]).contiguous()
+
+
+def from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None):
+    """Costs ``|x - y|^2`` (clamped at 0) of point-cloud pairs, built on the GPU.
+
+    xs[p]: (n_p, d), ys[p]: (m_p, d) float arrays or tensors (or (B, n, d)
+    tensors for equal sizes); regs: (B,).
+    With ``normalize`` each cost is taken relative to its maximum.
+    """
+    device = device or _device.require_gpu()
+    lib = _lib.load()
+    ns = [len(x) for x in xs]
+    ms = [len(y) for y in ys]
+    d = int(xs[0].shape[1])
+    ns, ms, total, moff, ns_d, ms_d, ld_d, aoff, boff = _layout(ns, ms, device)
+    X = _pack_points(xs, device)
+    Y = _pack_points(ys, device)
+    M = torch.empty(total, dtype=torch.float32, device=device)
+    cmax = torch.empty(len(ns), dtype=torch.float32, device=device)
+    rc = lib.potamd_sqeuclidean_ragged(
+        len(ns), int(ns.max()), int(ms.max()), d, _ptr(X), _ptr(Y), _ptr(aoff), _ptr(boff),
+        _ptr(ns_d), _ptr(ms_d), _ptr(M), _ptr(moff), _ptr(ld_d), _ptr(cmax),
+        ctypes.c_void_p(_device.stream_ptr(device)))
+    _lib.check(rc, "potamd_sqeuclidean_ragged")
+    regs_t = torch.as_tensor(np.asarray(regs, dtype=np.float64)).to(device)
+    if normalize:
+        regs_eff = regs_t * cmax.to(torch.float64)
+    else:
+        cmax.fill_(1.0)
+        regs_eff = regs_t
+    return RaggedProblems(device=device, ns=ns, ms=ms, M=M, moff=moff, ns_d=ns_d, ms_d=ms_d,
+                          ldms_d=ld_d, aoff=aoff, boff=boff, a=_weights(a, ns, device),
+                          b=_weights(b, ms, device), regs=regs_eff, cmax=cmax)
+
+
+def from_costs(Ms, regs, a=None, b=None, device=None):
+    """Pack given cost matrices (float32) without normalisation."""
+    device = device or _device.require_gpu()
+    ns = [int(np.shape(M)[0]) for M in Ms]
+    ms = [int(np.shape(M)[1]) for M in Ms]
+    ns, ms, total, moff, ns_d, ms_d, ld_d, aoff, boff = _layout(ns, ms, device)
+    host = np.zeros(total, dtype=np.float32)
+    off = 0
+    for M, n, m in zip(Ms, ns, ms):
+        ld = _round4(int(m))
+        blk = np.zeros((n, ld), dtype=np.float32)
+        blk[:, :m] = np.asarray(M, dtype=np.float32)
+        host[off:off + n * ld] = blk.ravel()
+        off += n * ld
+    Mt = torch.from_numpy(host).to(device)
+    regs_t = torch.as_tensor(np.asarray(regs, dtype=np.float64)).to(device)
+    return RaggedProblems(device=device, ns=ns, ms=ms, M=Mt, moff=moff, ns_d=ns_d, ms_d=ms_d,
+                          ldms_d=ld_d, aoff=aoff, boff=boff, a=_weights(a, ns, device),
+                          b=_weights(b, ms, device), regs=regs_t,
+                          cmax=torch.ones(len(ns), dtype=torch.float32, device=device))
+
+
+@dataclass
+class RaggedResult:
+    u: torch.Tensor          # (sum n,) float64 log potentials
+    v: torch.Tensor          # (sum m,) float64
+    niter: torch.Tensor      # (B,) int32, the reference's `ii` at the stop
+    status: torch.Tensor     # (B,) int32: 0 converged, 1 max-iter
+    err_first: torch.Tensor  # (B,) float64 marginal error at ii = 0
+    err_last: torch.Tensor   # (B,) float64 last checked marginal error
+
+    def split(self, R, p):
+        ao, bo = int(R.aoff[p].item()), int(R.boff[p].item())
+        return self.u[ao:ao + int(R.ns[p])], self.v[bo:bo + int(R.ms[p])]
+
+
+def launch_order(R, work=None):
+    """Problems by decreasing expected work (default n*m/reg), longest first."""
+    if work is None:
+        regs = R.regs.double().cpu().numpy() / R.cmax.double().cpu().numpy()
+        work = R.ns.astype(np.float64) * R.ms / regs
+    order = np.argsort(-np.asarray(work), kind="stable").astype(np.int32)
+    return torch.tensor(order, device=R.device)
+
+
+def sinkhorn_log_ragged(R, u0=None, numItermax=1000, stopThr=1e-9, order=None,
+                        fallback_thr=0.0, waves=0):
+    """Solve every problem of ``R`` with sinkhorn_log semantics in one launch.
+
+    u0: (sum n,) float64 initial log potentials (None = zeros, the cold start).
+    fallback_thr > 0: a problem whose first checked error exceeds it restarts
+    cold (its status gets bit 4).  Returns a RaggedResult; the call is
+    asynchronous on the current stream.
+    """
+    lib = _lib.load()
+    dev = R.device
+    B = R.batch
+    total_n = int(R.ns.sum())
+    u = torch.zeros(total_n, dtype=torch.float64, device=dev) if u0 is None \
+        else u0.to(device=dev, dtype=torch.float64).contiguous().clone()
+    v = torch.empty(int(R.ms.sum()), dtype=torch.float64, device=dev)
+    niter = torch.empty(B, dtype=torch.int32, device=dev)
+    status = torch.empty(B, dtype=torch.int32, device=dev)
+    e_first = torch.empty(B, dtype=torch.float64, device=dev)
+    e_last = torch.empty(B, dtype=torch.float64, device=dev)
+    if order is None:
+        order = launch_order(R)
+    rc = lib.potamd_sinkhorn_log_batch(
+        _lib.F32, B, int(R.ns.max()), int(R.ms.max()), _ptr(R.M), _ptr(R.moff), _ptr(R.ns_d),
+        _ptr(R.ms_d), _ptr(R.ldms_d), _ptr(R.aoff), _ptr(R.boff), _ptr(R.regs), _ptr(R.a),
+        _ptr(R.b), _ptr(u), _ptr(v), int(numItermax), float(stopThr), float(fallback_thr),
+        _ptr(order), 1, _ptr(niter), _ptr(status), _ptr(e_last), _ptr(e_first), int(waves),
+        ctypes.c_void_p(_device.stream_ptr(dev)))
+    _lib.check(rc, "potamd_sinkhorn_log_batch")
+    return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
+                        err_last=e_last)

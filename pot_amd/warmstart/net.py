@@ -1,0 +1,226 @@
+"""Warm-start network on the GPU: DeepSets encoder + per-point MLP.
+
+Per problem (sources x: n x 16, targets y: m x 16, target weights b, reg),
+with points scaled by 1/sqrt(max cost) so every problem is scale free:
+
+    e_j = [y_j, m b_j, log reg]      H1 = relu(e W_e1 + b_e1)
+    H2  = relu(H1 W_e2 + b_e2)       z  = mean_j H2
+    c   = z W_m1z + b_m1             G1 = relu(x W_m1x + c)
+    G2  = relu(G1 W_m2 + b_m2)       f  = G2 w_m3 + b_m3
+
+``[x_i, z] W_m1`` is split into ``x_i W_m1x + z W_m1z`` so the z term is
+computed once per problem.  f is the dual potential in units of the
+normalised cost; the solver starts from u0 = f / reg.
+
+Every forward/backward step runs in libpot_amd.so: the four wide layers on
+the bf16 MFMA GEMM (fp32 accumulation, fused bias / per-problem broadcast /
+ReLU / ReLU-mask epilogues), the rest on small fp32 kernels; weights are
+fp32 masters updated by the library's Adam kernel, with bf16 copies for the
+GEMMs.  The float64 NumPy restatement in oracle/warmnet.py is the checker.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _device, _lib
+
+H = 256
+DX = 16
+KIN = 32  # input width padded to one GEMM K slice
+
+# op codes of potamd_mlp_op
+OP_BUILD, OP_SEGMEAN, OP_VECMAT, OP_VECMATT, OP_HEAD, OP_LOSS = 0, 1, 2, 3, 4, 5
+OP_HEADBACK, OP_HEADWGRAD, OP_COLSUM, OP_SEGSUM, OP_POOLBACK, OP_OUTER = 6, 7, 8, 9, 10, 11
+OP_ADAM, OP_TOBF16 = 12, 13
+
+PARAM_SHAPES = {
+    "W_e1": (KIN, H), "b_e1": (H,), "W_e2": (H, H), "b_e2": (H,),
+    "W_m1x": (KIN, H), "W_m1z": (H, H), "b_m1": (H,), "W_m2": (H, H), "b_m2": (H,),
+    "w_m3": (H,), "b_m3": (1,),
+}
+GEMM_WEIGHTS = ("W_e1", "W_e2", "W_m1x", "W_m2")
+R0 = 1e-2
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class WarmStartNet:
+    def __init__(self, params, device):
+        self.device = device
+        self.w = {k: torch.as_tensor(np.asarray(v, dtype=np.float32)).to(device).contiguous()
+                  for k, v in params.items()}
+        self.wb = {k: torch.empty(self.w[k].shape, dtype=torch.bfloat16, device=device)
+                   for k in GEMM_WEIGHTS}
+        self.g = {k: torch.zeros_like(v) for k, v in self.w.items()}
+        self.m1 = {k: torch.zeros_like(v) for k, v in self.w.items()}
+        self.m2 = {k: torch.zeros_like(v) for k, v in self.w.items()}
+        self.step_count = 0
+        self.lib = _lib.load()
+        self.sync_bf16()
+
+    # ------------------------------------------------------------------ io
+    @staticmethod
+    def init_params(seed=0):
+        """He-initialised parameters (padded input rows are zero)."""
+        rng = np.random.default_rng(seed)
+
+        def lin(fan_in, rows, cols):
+            return rng.standard_normal((rows, cols)) * np.sqrt(2.0 / fan_in)
+        p = {k: np.zeros(s) for k, s in PARAM_SHAPES.items()}
+        p["W_e1"][:DX + 2] = lin(DX + 2, DX + 2, H)
+        p["W_e2"] = lin(H, H, H)
+        p["W_m1x"][:DX] = lin(DX + H, DX, H)
+        p["W_m1z"] = lin(DX + H, H, H)
+        p["W_m2"] = lin(H, H, H)
+        p["w_m3"] = lin(H, H, 1)[:, 0] * 0.1
+        return p
+
+    @classmethod
+    def load(cls, path, device):
+        with np.load(path, allow_pickle=False) as z:
+            params = {k: z[k] for k in PARAM_SHAPES}
+        return cls(params, device)
+
+    def save(self, path, **meta):
+        arrs = {k: v.detach().cpu().numpy() for k, v in self.w.items()}
+        arrs.update({k: np.asarray(v) for k, v in meta.items()})
+        np.savez(path, **arrs)
+
+    def params_numpy(self):
+        return {k: v.detach().cpu().double().numpy() for k, v in self.w.items()}
+
+    # ------------------------------------------------------------- kernels
+    def _stream(self):
+        return ctypes.c_void_p(_device.stream_ptr(self.device))
+
+    def _op(self, op, rows=0, seg_len=1, P=0, k=0, n=0, count=0, flag=0, scalar=0.0, hp=None,
+            x=None, y=None, z=None, w=None, out=None, out2=None):
+        hp_arr = None
+        if hp is not None:
+            hp_arr = (ctypes.c_float * 7)(*hp)
+        rc = self.lib.potamd_mlp_op(op, rows, seg_len, P, k, n, count, flag, float(scalar),
+                                    hp_arr, _p(x), _p(y), _p(z), _p(w), _p(out), _p(out2),
+                                    self._stream())
+        _lib.check(rc, f"potamd_mlp_op({op})")
+
+    def _gemm(self, A, B, C, M, N, K, *, transA=0, transB=0, out_kind=0, bias=None,
+              rowvec=None, seg_len=1, relu=0, mask=None, splits=1, lda=None, ldb=None):
+        lda = lda or (M if transA else K)
+        ldb = ldb or (K if transB else N)
+        rc = self.lib.potamd_gemm_bf16(M, N, K, _p(A), lda, transA, _p(B), ldb, transB, _p(C), N,
+                                       out_kind, 1.0, _p(bias), _p(rowvec), seg_len, relu,
+                                       _p(mask), N, splits, self._stream())
+        _lib.check(rc, "potamd_gemm_bf16")
+
+    def sync_bf16(self):
+        for k in GEMM_WEIGHTS:
+            self._op(OP_TOBF16, count=self.w[k].numel(), x=self.w[k], out=self.wb[k])
+
+    # ------------------------------------------------------------- inputs
+    def build_inputs(self, pts, P, seg_len, b=None, regs=None, cmax=None, extra=False):
+        """(P*seg_len, KIN) bf16 rows: scaled points (+ [m b, log reg] if extra)."""
+        rows = P * seg_len
+        out = torch.empty((rows, KIN), dtype=torch.bfloat16, device=self.device)
+        self._op(OP_BUILD, rows=rows, seg_len=seg_len, n=DX, k=KIN, flag=int(extra),
+                 x=pts.contiguous(), y=b, z=regs, w=cmax, out=out)
+        return out
+
+    # ------------------------------------------------------------ forward
+    def forward(self, Xin, Ein, P, n, m, keep=False):
+        dev = self.device
+        w, wb = self.w, self.wb
+        H1 = torch.empty((P * m, H), dtype=torch.bfloat16, device=dev)
+        self._gemm(Ein, wb["W_e1"], H1, P * m, H, KIN, bias=w["b_e1"], relu=1)
+        H2 = torch.empty_like(H1)
+        self._gemm(H1, wb["W_e2"], H2, P * m, H, H, bias=w["b_e2"], relu=1)
+        z = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._op(OP_SEGMEAN, seg_len=m, P=P, k=H, x=H2, out=z)
+        c = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._op(OP_VECMAT, P=P, k=H, n=H, x=z, w=w["W_m1z"], y=w["b_m1"], out=c)
+        G1 = torch.empty((P * n, H), dtype=torch.bfloat16, device=dev)
+        self._gemm(Xin, wb["W_m1x"], G1, P * n, H, KIN, rowvec=c, seg_len=n, relu=1)
+        G2 = torch.empty_like(G1)
+        self._gemm(G1, wb["W_m2"], G2, P * n, H, H, bias=w["b_m2"], relu=1)
+        f = torch.empty(P * n, dtype=torch.float32, device=dev)
+        self._op(OP_HEAD, rows=P * n, k=H, x=G2, w=w["w_m3"], y=w["b_m3"], out=f)
+        if keep:
+            self._cache = (Xin, Ein, P, n, m, H1, H2, z, G1, G2)
+        return f
+
+    # ----------------------------------------------------------- backward
+    def loss_grad(self, f, target, regs, P, n):
+        """Centred weighted MSE (oracle.warmnet.loss) and d loss / d f."""
+        df = torch.empty_like(f)
+        lp = torch.empty(P, dtype=torch.float32, device=self.device)
+        self._op(OP_LOSS, seg_len=n, P=P, scalar=R0, x=f, y=target, z=regs, out=df, out2=lp)
+        return lp, df
+
+    def backward(self, df):
+        Xin, Ein, P, n, m, H1, H2, z, G1, G2 = self._cache
+        dev = self.device
+        w, wb, g = self.w, self.wb, self.g
+        for t in g.values():
+            t.zero_()
+        rows_n, rows_m = P * n, P * m
+        self._op(OP_HEADWGRAD, rows=rows_n, k=H, x=G2, y=df, out=g["w_m3"], out2=g["b_m3"])
+        dB2 = torch.empty((rows_n, H), dtype=torch.bfloat16, device=dev)
+        self._op(OP_HEADBACK, rows=rows_n, k=H, x=df, w=w["w_m3"], y=G2, out=dB2)
+        splits = max(1, min(64, rows_n // 512))
+        self._gemm(G1, dB2, g["W_m2"], H, H, rows_n, transA=1, out_kind=2, splits=splits)
+        self._op(OP_COLSUM, rows=rows_n, k=H, x=dB2, out=g["b_m2"])
+        dB1 = torch.empty_like(dB2)
+        self._gemm(dB2, wb["W_m2"], dB1, rows_n, H, H, transB=1, mask=G1)
+        self._gemm(Xin, dB1, g["W_m1x"], KIN, H, rows_n, transA=1, out_kind=2, splits=splits)
+        dc = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._op(OP_SEGSUM, seg_len=n, P=P, k=H, x=dB1, out=dc)
+        ones = torch.ones((P, 1), dtype=torch.float32, device=dev)
+        self._op(OP_OUTER, P=P, k=1, n=H, x=ones, y=dc, out=g["b_m1"])
+        self._op(OP_OUTER, P=P, k=H, n=H, x=z, y=dc, out=g["W_m1z"])
+        dz = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._op(OP_VECMATT, P=P, k=H, n=H, x=dc, w=w["W_m1z"], out=dz)
+        dA2 = torch.empty((rows_m, H), dtype=torch.bfloat16, device=dev)
+        self._op(OP_POOLBACK, rows=rows_m, seg_len=m, k=H, x=dz, y=H2, out=dA2)
+        splits_m = max(1, min(64, rows_m // 512))
+        self._gemm(H1, dA2, g["W_e2"], H, H, rows_m, transA=1, out_kind=2, splits=splits_m)
+        self._op(OP_COLSUM, rows=rows_m, k=H, x=dA2, out=g["b_e2"])
+        dA1 = torch.empty_like(dA2)
+        self._gemm(dA2, wb["W_e2"], dA1, rows_m, H, H, transB=1, mask=H1)
+        self._gemm(Ein, dA1, g["W_e1"], KIN, H, rows_m, transA=1, out_kind=2, splits=splits_m)
+        self._op(OP_COLSUM, rows=rows_m, k=H, x=dA1, out=g["b_e1"])
+        return g
+
+    def adam_step(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        self.step_count += 1
+        t = self.step_count
+        bc1, bc2 = 1 - b1 ** t, 1 - b2 ** t
+        for k, wt in self.w.items():
+            out2 = self.wb[k] if k in self.wb else None
+            self._op(OP_ADAM, count=wt.numel(), hp=(lr, b1, b2, eps, bc1, bc2, wd),
+                     x=self.g[k], y=self.m1[k], z=self.m2[k], out=wt, out2=out2)
+        # padded input rows of the first layers stay exactly zero
+        self.w["W_e1"][DX + 2:].zero_()
+        self.w["W_m1x"][DX:].zero_()
+        self.wb["W_e1"][DX + 2:].zero_()
+        self.wb["W_m1x"][DX:].zero_()
+
+    # ---------------------------------------------------------- inference
+    def predict_f(self, X, Y, b, regs, cmax, P, n, m):
+        """f (P*n,) float32 for uniform-size problems; X: (P*n, 16) fp32."""
+        Xin = self.build_inputs(X, P, n, cmax=cmax)
+        Ein = self.build_inputs(Y, P, m, b=b, regs=regs, cmax=cmax, extra=True)
+        return self.forward(Xin, Ein, P, n, m)
+
+    def predict_u0(self, R, X, Y, regs):
+        """Initial log potentials u0 = f / reg for the ragged batch R (equal sizes)."""
+        P = R.batch
+        n, m = int(R.ns[0]), int(R.ms[0])
+        if not (np.all(R.ns == n) and np.all(R.ms == m)):
+            raise ValueError("the warm-start network expects problems of one size")
+        regs_t = torch.as_tensor(np.asarray(regs, dtype=np.float64)).to(self.device)
+        f = self.predict_f(X.reshape(-1, DX).float(), Y.reshape(-1, DX).float(), R.b, regs_t,
+                           R.cmax, P, n, m)
+        return (f.double().view(P, n) / regs_t[:, None]).reshape(-1)
