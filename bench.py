@@ -35,6 +35,10 @@ def _args():
     ap.add_argument("--workload", default="warmstart", choices=["warmstart", "dense", "ragged"])
     ap.add_argument("--batch", type=int, default=4096, help="problems per step")
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per problem (0 = default)")
+    ap.add_argument("--prefetch", type=int, default=0, help="rows prefetched per wave (0 = 1)")
+    ap.add_argument("--check-every", type=int, default=1,
+                    help="iterations between marginal checks (1: stop at the first iterate "
+                         "with err <= stopThr; 10: the reference's cadence)")
     ap.add_argument("--stop", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -82,6 +86,8 @@ def run_warmstart(args, torch, dev):
             e0.record()
             res = ragged.sinkhorn_log_ragged(R, u0=u0, numItermax=args.max_iter,
                                              stopThr=args.stop, waves=args.waves,
+                                             prefetch=args.prefetch,
+                                             check_every=args.check_every,
                                              fallback_thr=ws_mod.FALLBACK_THR if warm else 0.0)
             e1.record()
             torch.cuda.synchronize()
@@ -120,6 +126,7 @@ def run_warmstart(args, torch, dev):
         "config": {"workload": "warmstart-family n=m=1024 d=16 GMM(1-8) reg~logU[5e-3,5e-2]",
                    "problems_per_step": B, "stopThr": args.stop, "numItermax": args.max_iter,
                    "start": "learned" if warm else "cold (network weights absent)",
+                   "check_every": args.check_every,
                    "seed_family": 1_000_000},
         "cold": cold,
         "warm": warm,

@@ -74,7 +74,8 @@ size_t potamd_dense_workspace_size(int dtype, int batch, int n, int m, int rows_
  *    histograms as a batch sharing M through bstride = 0).
  * err_hist: float64 device array of err_cap entries, the reference's
  *    log["err"] (one value per check).
- * threads, cols_per_thread, rows_per_block: tile shape, 0 = automatic.
+ * threads, cols_per_thread, rows_per_block, prefetch (rows loaded ahead, 1
+ *    or 2): tile shape, 0 = automatic.
  * host_ctrl: pinned host memory of >= 512 bytes used for asynchronous stop
  *    polling, or NULL (then each check synchronises).
  * info (host, 4 ints): [0] niter (the reference's `ii`), [1] number of err
@@ -85,8 +86,8 @@ int potamd_sinkhorn_dense(int dtype, int domain, int order, int kernel_given,
                           double reg, const double* a, const double* b, double* u, double* v,
                           int max_iter, double stop_thr, int err_norm, double* err_hist,
                           int err_cap, int threads, int cols_per_thread, int rows_per_block,
-                          void* workspace, size_t ws_bytes, void* host_ctrl, void* stream,
-                          int* info);
+                          int prefetch, void* workspace, size_t ws_bytes, void* host_ctrl,
+                          void* stream, int* info);
 
 /* Bytes of workspace needed by potamd_plan_dense. */
 size_t potamd_plan_workspace_size(int batch, int n);
@@ -145,6 +146,9 @@ int potamd_sqeuclidean_ragged(int batch, int max_n, int max_m, int d, const floa
  *    max_m >= every m_p (both <= 1024).
  * a, u: problem p's rows at offset aoff[p]; b, v: columns at boff[p].
  *    u holds u0 on entry (zeros = the reference's cold start).
+ * check_every: iterations between marginal checks (0 = 10, the reference's
+ *    cadence; 1 stops a problem at the first iteration that meets stopThr --
+ *    the error is formed every sweep anyway).
  * order: launch order of the problems (e.g. most work first) or NULL.
  * vec_aligned: 1 if M is 16-byte aligned and every moff, ldms is a multiple
  *    of 4 (vector loads), else 0.
@@ -152,14 +156,16 @@ int potamd_sqeuclidean_ragged(int batch, int max_n, int max_m, int d, const floa
  *    checked error exceeds it restarts from u0 = 0 (status gets bit 4).
  * niter, status: per-problem outputs (device int arrays); err_last / err_first:
  *    last and first checked marginal errors (device, may be NULL).
- * waves: wavefronts per problem (4 or 8; 0 = default).
+ * waves: wavefronts per problem (4 or 8; 0 = default) in the low byte, rows
+ *    prefetched per wave (1 or 2, 0 = 1; 2 needs max_m > 512) in the next byte.
  */
 int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int max_m, const void* M,
                               const int64_t* moff, const int* ns, const int* ms, const int* ldms,
                               const int64_t* aoff, const int64_t* boff, const double* regs,
                               const double* a, const double* b, double* u, double* v,
-                              int max_iter, double stop_thr, double fallback_thr,
-                              const int* order, int vec_aligned, int* niter, int* status,
+                              int max_iter, int check_every, double stop_thr,
+                              double fallback_thr, const int* order, int vec_aligned,
+                              int* niter, int* status,
                               double* err_last, double* err_first, int waves, void* stream);
 
 /* ---- learned warm start (no counterpart in the reference) ------------- */
@@ -167,7 +173,8 @@ int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int max_m, const 
 /* bf16 MFMA GEMM with fused epilogue, the linear layers of the warm-start
  * network (forward Y = X W, backward dX = dY W^T, dW = X^T dY):
  *   C[M x N] = alpha * op(A)[M x K] op(B)[K x N]  (+ bias[N])
- *              (+ rowvec[i / seg_len][N]) (relu) (* (mask[i][j] > 0))
+ *              (+ rowvec[i / seg_len][N]) (leaky relu, slope 0.01)
+ *              (* (mask[i][j] > 0 ? 1 : 0.01))
  * op(A)[i][k] = transA ? A[k*lda + i] : A[i*lda + k]; op(B) likewise with
  * op(B)[k][j] = transB ? B[j*ldb + k] : B[k*ldb + j].  A, B, mask bf16.
  * out_kind: 0 bf16 store, 1 fp32 store, 2 fp32 atomic add (split-K over
@@ -184,7 +191,8 @@ int potamd_gemm_bf16(int M, int N, int K, const void* A, int64_t lda, int transA
  *  0 build inputs, 1 segment mean, 2 vector x matrix, 3 vector x matrix^T,
  *  4 output head, 5 centred weighted loss + gradient, 6 head backward,
  *  7 head weight gradient, 8 column sums, 9 segment sums, 10 pooling
- *  backward, 11 outer-product accumulation, 12 Adam step, 13 fp32 -> bf16.
+ *  backward, 11 outer-product accumulation, 12 Adam step, 13 fp32 -> bf16,
+ *  14 scaled squared row norms.
  * hp: 7 floats of hyper-parameters (Adam) or NULL.
  */
 int potamd_mlp_op(int op, int rows, int seg_len, int P, int k, int n, int64_t count, int flag,

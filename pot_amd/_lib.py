@@ -28,7 +28,7 @@ SIGNATURES = {
         c_vp, c_i64, c_i64, c_int, c_int, c_int,  # M, ldm, bstride, batch, n, m
         c_dbl, c_vp, c_vp, c_vp, c_vp,       # reg, a, b, u, v
         c_int, c_dbl, c_int, c_vp, c_int,    # max_iter, stop_thr, err_norm, err_hist, err_cap
-        c_int, c_int, c_int,                 # threads, cols_per_thread, rows_per_block
+        c_int, c_int, c_int, c_int,          # threads, cols_per_thread, rows_per_block, prefetch
         c_vp, c_size, c_vp, c_vp, c_ip]),    # workspace, ws_bytes, host_ctrl, stream, info
     "potamd_plan_workspace_size": (c_size, [c_int, c_int]),
     "potamd_plan_dense": (c_int, [
@@ -39,7 +39,7 @@ SIGNATURES = {
         c_vp, c_vp, c_vp, c_vp,              # moff, ns, ms, ldms
         c_vp, c_vp, c_vp,                    # aoff, boff, regs
         c_vp, c_vp, c_vp, c_vp,              # a, b, u, v
-        c_int, c_dbl, c_dbl, c_vp, c_int,    # max_iter, stop_thr, fallback_thr, order, vec
+        c_int, c_int, c_dbl, c_dbl, c_vp, c_int,  # max_iter, check_every, stop, fallback, order, vec
         c_vp, c_vp, c_vp, c_vp,              # niter, status, err_last, err_first
         c_int, c_vp]),                       # waves, stream
     "potamd_sqeuclidean_ragged": (c_int, [

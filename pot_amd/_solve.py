@@ -45,7 +45,10 @@ def dense_solve(M, a, b, u0, v0, reg, *, domain, order, kernel_given=False,
         bs = M.stride(0) if bstride is None else bstride
     if M.stride(-1) != 1:
         raise ValueError("cost rows must be contiguous")
-    T, cpt, rpb = tile or (0, 0, 0)
+    if tile is None:
+        from . import autotune
+        tile = autotune.lookup("f32" if code == _lib.F32 else "f64", n, m)
+    T, cpt, rpb, pf = (tuple(tile) + (0,) * 4)[:4] if tile else (0, 0, 0, 0)
     ws_bytes = lib.potamd_dense_workspace_size(code, B, n, m, rpb)
     if ws_bytes == 0:
         raise _lib.LibraryError("invalid shape for the dense solver")
@@ -61,7 +64,7 @@ def dense_solve(M, a, b, u0, v0, reg, *, domain, order, kernel_given=False,
         _ptr(M), ldm, bs, B, n, m, float(reg) if reg is not None else 1.0,
         _ptr(a), _ptr(b), _ptr(u), _ptr(v),
         int(max_iter), float(stop_thr), int(err_norm), _ptr(err_hist), err_cap,
-        T, cpt, rpb, _ptr(ws), ws_bytes, ctypes.c_void_p(hc.data_ptr()),
+        T, cpt, rpb, pf, _ptr(ws), ws_bytes, ctypes.c_void_p(hc.data_ptr()),
         ctypes.c_void_p(_device.stream_ptr(dev)), info)
     _lib.check(rc, "potamd_sinkhorn_dense")
     if rc == _lib.NEED_EXACT:

@@ -27,7 +27,7 @@ namespace potamd {
 
 constexpr int kBatchMaxN = 1024;
 
-template <int CPT, int VEC, int W>
+template <int CPT, int VEC, int W, int PF>
 __global__ __launch_bounds__(W * kWave) void batch_log_kernel(BatchArgs A) {
   constexpr int T = W * kWave;
   constexpr int NCH = CPT / VEC;
@@ -137,7 +137,8 @@ restart:
     __syncthreads();
   }
 
-  float cur[CPT], nxt[CPT];
+  // Rows in flight per wave: the current one plus PF prefetched.
+  float cur[CPT], nxt[CPT], nx2[PF > 1 ? CPT : 1];
   for (int ii = 0; ii < A.max_iter; ++ii) {
     // ---- row sweep: u from v, column sums of P(u_new, v) ----
     float acc[CPT];
@@ -145,8 +146,15 @@ restart:
     for (int q = 0; q < CPT; ++q) acc[q] = 0.f;
     int i = wid;
     if (i < n) load_row(i, cur);
-    for (; i < n; i += W) {
+    if constexpr (PF > 1) {
       if (i + W < n) load_row(i + W, nxt);
+    }
+    for (; i < n; i += W) {
+      if constexpr (PF > 1) {
+        if (i + 2 * W < n) load_row(i + 2 * W, nx2);
+      } else {
+        if (i + W < n) load_row(i + W, nxt);
+      }
       const float a_i = af[i];
       if (a_i > 0.f) {
         double c2 = c2s[i];
@@ -186,6 +194,10 @@ restart:
       }
 #pragma unroll
       for (int q = 0; q < CPT; ++q) cur[q] = nxt[q];
+      if constexpr (PF > 1) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) nxt[q] = nx2[q];
+      }
     }
 #pragma unroll
     for (int q = 0; q < CPT; ++q) part[wid][q * kWave + lane] = acc[q];
@@ -221,7 +233,7 @@ restart:
       for (int w = 0; w < W; ++w) t += red[w];
       const double err = sqrt(t);
       int stop = 0;
-      if (ii % 10 == 0) {
+      if (ii % A.check_every == 0) {
         err_last = err;
         if (ii == 0) {
           err_first = err;
@@ -273,25 +285,27 @@ int batch_cpt(int max_m) {
   return 0;
 }
 
-int batch_waves_default() { return 4; }
+int batch_waves_default() { return 8; }
 
-hipError_t launch_batch_log(int cpt, int vec_ok, int waves, const BatchArgs& a, int B,
+hipError_t launch_batch_log(int cpt, int vec_ok, int waves, int pf, const BatchArgs& a, int B,
                             hipStream_t s) {
   dim3 grid(B);
-#define POTAMD_BATCH_CASE(C, WV)                                                      \
-  if (cpt == C && waves == WV) {                                                     \
-    if (vec_ok)                                                                      \
-      hipLaunchKernelGGL((batch_log_kernel<C, 4, WV>), grid, dim3(WV * kWave), 0, s, a); \
-    else                                                                             \
-      hipLaunchKernelGGL((batch_log_kernel<C, 1, WV>), grid, dim3(WV * kWave), 0, s, a); \
-    return hipGetLastError();                                                        \
+#define POTAMD_BATCH_CASE(C, WV, PP)                                                          \
+  if (cpt == C && waves == WV && pf == PP) {                                                 \
+    if (vec_ok)                                                                              \
+      hipLaunchKernelGGL((batch_log_kernel<C, 4, WV, PP>), grid, dim3(WV * kWave), 0, s, a); \
+    else                                                                                     \
+      hipLaunchKernelGGL((batch_log_kernel<C, 1, WV, PP>), grid, dim3(WV * kWave), 0, s, a); \
+    return hipGetLastError();                                                                \
   }
-  POTAMD_BATCH_CASE(4, 4)
-  POTAMD_BATCH_CASE(8, 4)
-  POTAMD_BATCH_CASE(16, 4)
-  POTAMD_BATCH_CASE(4, 8)
-  POTAMD_BATCH_CASE(8, 8)
-  POTAMD_BATCH_CASE(16, 8)
+  POTAMD_BATCH_CASE(4, 4, 1)
+  POTAMD_BATCH_CASE(8, 4, 1)
+  POTAMD_BATCH_CASE(16, 4, 1)
+  POTAMD_BATCH_CASE(4, 8, 1)
+  POTAMD_BATCH_CASE(8, 8, 1)
+  POTAMD_BATCH_CASE(16, 8, 1)
+  POTAMD_BATCH_CASE(16, 4, 2)
+  POTAMD_BATCH_CASE(16, 8, 2)
 #undef POTAMD_BATCH_CASE
   return hipErrorInvalidConfiguration;
 }

@@ -138,7 +138,8 @@ extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kerne
                                      int n, int m, double reg, const double* a, const double* b,
                                      double* u, double* v, int max_iter, double stop_thr,
                                      int err_norm, double* err_hist, int err_cap, int threads,
-                                     int cols_per_thread, int rows_per_block, void* workspace,
+                                     int cols_per_thread, int rows_per_block, int prefetch,
+                                     void* workspace,
                                      size_t ws_bytes, void* host_ctrl, void* stream, int* info) {
   if ((dtype != POTAMD_F32 && dtype != POTAMD_F64) || (domain != POTAMD_LOG && domain != POTAMD_SCALING) ||
       (order != POTAMD_COL_FIRST && order != POTAMD_ROW_FIRST) || !M || !a || !b || !u || !v ||
@@ -148,9 +149,15 @@ extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kerne
     return POTAMD_EINVAL;
   const int rpb = rows_per_block > 0 ? rows_per_block : default_rpb(n);
   if (rpb > 256) return POTAMD_EINVAL;
-  int T = threads, cpt = cols_per_thread;
-  if (T == 0 || cpt == 0) row_config_default(dtype, m, &T, &cpt);
-  if (T == 0 || !row_config_valid(dtype, T, cpt) || (int64_t)T * cpt < m) return POTAMD_EUNSUPPORTED;
+  int T = threads, cpt = cols_per_thread, pf = prefetch;
+  if (T == 0 || cpt == 0) {
+    int pf_default = 1;
+    row_config_default(dtype, m, &T, &cpt, &pf_default);
+    if (pf == 0) pf = pf_default;
+  }
+  if (pf == 0) pf = 1;
+  if (T == 0 || !row_config_valid(dtype, T, cpt, pf) || (int64_t)T * cpt < m)
+    return POTAMD_EUNSUPPORTED;
   const size_t es = dtype == POTAMD_F32 ? 4 : 8;
   const int vec = dtype == POTAMD_F32 ? 4 : 2;
   const int vec_ok = ((uintptr_t)M % 16 == 0) && (ldm % vec == 0) && (bstride % vec == 0) &&
@@ -241,7 +248,7 @@ extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kerne
       ra.u_old = L.U[0];
       ra.u_new = L.U[1];
       ra.v = L.V[0];
-      HIP_TRY(launch_row_sweep(dtype, vec_ok, T, cpt, domain, 1, ra, G, batch, s));
+      HIP_TRY(launch_row_sweep(dtype, vec_ok, T, cpt, pf, domain, 1, ra, G, batch, s));
       ma.iter = -1;
       ma.prologue = 1;
       ma.v_cur = L.V[0];
@@ -266,7 +273,7 @@ extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kerne
     ra.u_old = L.U[ii & 1];
     ra.u_new = L.U[(ii + 1) & 1];
     ra.v = L.V[(ii + vo) & 1];
-    if (launch_row_sweep(dtype, vec_ok, T, cpt, domain, 0, ra, G, batch, s) != hipSuccess) {
+    if (launch_row_sweep(dtype, vec_ok, T, cpt, pf, domain, 0, ra, G, batch, s) != hipSuccess) {
       rc = POTAMD_EHIP;
       break;
     }
@@ -402,12 +409,14 @@ extern "C" int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int ma
                                          const int* ms, const int* ldms, const int64_t* aoff,
                                          const int64_t* boff, const double* regs,
                                          const double* a, const double* b, double* u,
-                                         double* v, int max_iter, double stop_thr,
-                                         double fallback_thr, const int* order, int vec_aligned,
+                                         double* v, int max_iter, int check_every,
+                                         double stop_thr, double fallback_thr, const int* order,
+                                         int vec_aligned,
                                          int* niter, int* status, double* err_last,
                                          double* err_first, int waves, void* stream) {
   if (batch < 1 || max_n < 1 || max_m < 1 || !M || !moff || !ns || !ms || !ldms || !aoff ||
-      !boff || !regs || !a || !b || !u || !v || max_iter < 1 || !niter || !status)
+      !boff || !regs || !a || !b || !u || !v || max_iter < 1 || !niter || !status ||
+      check_every < 0)
     return POTAMD_EINVAL;
   if (dtype != POTAMD_F32) return POTAMD_EUNSUPPORTED;
   const int cpt = batch_cpt(max_m);
@@ -428,6 +437,7 @@ extern "C" int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int ma
   ba.v = v;
   ba.order = order;
   ba.max_iter = max_iter;
+  ba.check_every = check_every > 0 ? check_every : 10;
   ba.stop_thr = stop_thr;
   ba.fallback_thr = fallback_thr;
   ba.niter = niter;
@@ -435,9 +445,15 @@ extern "C" int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int ma
   ba.err_last = err_last;
   ba.err_first = err_first;
   const int vec_ok = vec_aligned && ((uintptr_t)M % 16 == 0);
+  // waves: low byte = wavefronts per problem (4 or 8), next byte = rows
+  // prefetched per wave (1 or 2; 2 only with 1024-column tiles).
+  int pf = (waves >> 8) & 0xff;
+  waves &= 0xff;
   if (waves == 0) waves = batch_waves_default();
-  if (waves != 4 && waves != 8) return POTAMD_EINVAL;
-  HIP_TRY(launch_batch_log(cpt, vec_ok, waves, ba, batch, static_cast<hipStream_t>(stream)));
+  if (pf == 0) pf = 1;
+  if ((waves != 4 && waves != 8) || (pf != 1 && pf != 2) || (pf == 2 && cpt != 16))
+    return POTAMD_EINVAL;
+  HIP_TRY(launch_batch_log(cpt, vec_ok, waves, pf, ba, batch, static_cast<hipStream_t>(stream)));
   return 0;
 }
 
@@ -504,10 +520,10 @@ extern "C" int potamd_mlp_op(int op, int rows, int seg_len, int P, int k, int n,
                              int flag, double scalar, const float* hp, const void* x,
                              const void* y, const void* z, const void* w, void* out, void* out2,
                              void* stream) {
-  if (op < 0 || op > kOpToBf16 || !out) return POTAMD_EINVAL;
+  if (op < 0 || op > kOpRowNorm || !out) return POTAMD_EINVAL;
   if ((op == kOpColSum || op == kOpHeadWGrad) && (k < 1 || k > 1024)) return POTAMD_EINVAL;
   if ((op == kOpSegmentMean || op == kOpSegmentSum || op == kOpPoolBack || op == kOpLoss ||
-       op == kOpBuildInputs) && seg_len < 1)
+       op == kOpBuildInputs || op == kOpRowNorm) && seg_len < 1)
     return POTAMD_EINVAL;
   MlpOp o;
   o.op = op;

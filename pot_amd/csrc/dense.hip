@@ -78,7 +78,7 @@ __device__ __forceinline__ void load_vec(const TM* p, TM* out) {
 // registers of the fp64 log2 do not add to the loop's live set.
 constexpr int kMaxRowsPerBlock = 256;
 
-template <typename TM, int T, int CPT, int VEC, int DOMAIN, int MODE>
+template <typename TM, int T, int CPT, int VEC, int DOMAIN, int MODE, int PF>
 __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
   using R = typename Real<TM>::type;
   constexpr int W = T / kWave;
@@ -124,7 +124,8 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
   const int r0 = g * A.rpb;
   const int r1 = min(n, r0 + A.rpb);
 
-  R cur[CPT], nxt[CPT];
+  // Rows in flight: the current one plus PF prefetched (rotating registers).
+  R cur[CPT], nxt[CPT], nx2[PF > 1 ? CPT : 1];
   auto load_row = [&](int i, R* dst) {
     const TM* row = Mp + (int64_t)i * A.ldm;
 #pragma unroll
@@ -142,12 +143,19 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
     }
   };
   if (r0 < r1) load_row(r0, cur);
+  if constexpr (PF > 1) {
+    if (r0 + 1 < r1) load_row(r0 + 1, nxt);
+  }
 
   const R s_hi = (R)(sizeof(TM) == 4 ? (double)A.s_hi : A.s2);
   const R s_lo = (R)(sizeof(TM) == 4 ? (double)A.s_lo : 0.0);
 
   for (int i = r0; i < r1; ++i) {
-    if (i + 1 < r1) load_row(i + 1, nxt);
+    if constexpr (PF > 1) {
+      if (i + 2 < r1) load_row(i + 2, nx2);
+    } else {
+      if (i + 1 < r1) load_row(i + 1, nxt);
+    }
     const double a_i = ap[i];
     if (DOMAIN == kLog) {
       if (a_i == 0.0) {
@@ -217,6 +225,10 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
     }
 #pragma unroll
     for (int q = 0; q < CPT; ++q) cur[q] = nxt[q];
+    if constexpr (PF > 1) {
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) nxt[q] = nx2[q];
+    }
   }
 
   // Column partials of this row block.
@@ -487,37 +499,44 @@ namespace potamd {
 
 namespace {
 
-template <typename TM, int T, int CPT, int VEC>
+template <typename TM, int T, int CPT, int VEC, int PF>
 hipError_t launch_row_t(int domain, int mode, const RowSweepArgs& a, dim3 grid, hipStream_t s) {
   if (domain == kLog) {
-    hipLaunchKernelGGL((row_sweep_kernel<TM, T, CPT, VEC, kLog, 0>), grid, dim3(T), 0, s, a);
+    hipLaunchKernelGGL((row_sweep_kernel<TM, T, CPT, VEC, kLog, 0, PF>), grid, dim3(T), 0, s, a);
   } else if (mode == 0) {
-    hipLaunchKernelGGL((row_sweep_kernel<TM, T, CPT, VEC, kScaling, 0>), grid, dim3(T), 0, s, a);
+    hipLaunchKernelGGL((row_sweep_kernel<TM, T, CPT, VEC, kScaling, 0, PF>), grid, dim3(T), 0, s,
+                       a);
   } else {
-    hipLaunchKernelGGL((row_sweep_kernel<TM, T, CPT, VEC, kScaling, 1>), grid, dim3(T), 0, s, a);
+    hipLaunchKernelGGL((row_sweep_kernel<TM, T, CPT, VEC, kScaling, 1, PF>), grid, dim3(T), 0, s,
+                       a);
   }
   return hipGetLastError();
 }
 
 template <typename TM, int VEC>
-hipError_t launch_row_v(int T, int cpt, int domain, int mode, const RowSweepArgs& a, dim3 grid,
-                        hipStream_t s) {
-#define POTAMD_ROW_CASE(TT, CC)                                              \
-  if (T == TT && cpt == CC) return launch_row_t<TM, TT, CC, VEC>(domain, mode, a, grid, s);
+hipError_t launch_row_v(int T, int cpt, int pf, int domain, int mode, const RowSweepArgs& a,
+                        dim3 grid, hipStream_t s) {
+#define POTAMD_ROW_CASE(TT, CC, PP)                                                   \
+  if (T == TT && cpt == CC && pf == PP)                                              \
+    return launch_row_t<TM, TT, CC, VEC, PP>(domain, mode, a, grid, s);
   if constexpr (sizeof(TM) == 4) {
-    POTAMD_ROW_CASE(64, 4)
-    POTAMD_ROW_CASE(256, 4)
-    POTAMD_ROW_CASE(256, 8)
-    POTAMD_ROW_CASE(512, 8)
-    POTAMD_ROW_CASE(1024, 8)
-    POTAMD_ROW_CASE(1024, 16)
+    POTAMD_ROW_CASE(64, 4, 1)
+    POTAMD_ROW_CASE(256, 4, 1)
+    POTAMD_ROW_CASE(256, 8, 1)
+    POTAMD_ROW_CASE(512, 8, 1)
+    POTAMD_ROW_CASE(1024, 8, 1)
+    POTAMD_ROW_CASE(1024, 16, 1)
+    POTAMD_ROW_CASE(1024, 8, 2)
+    POTAMD_ROW_CASE(1024, 16, 2)
+    POTAMD_ROW_CASE(512, 32, 1)
+    POTAMD_ROW_CASE(512, 32, 2)
   } else {
-    POTAMD_ROW_CASE(64, 2)
-    POTAMD_ROW_CASE(256, 2)
-    POTAMD_ROW_CASE(256, 4)
-    POTAMD_ROW_CASE(512, 4)
-    POTAMD_ROW_CASE(1024, 4)
-    POTAMD_ROW_CASE(1024, 8)
+    POTAMD_ROW_CASE(64, 2, 1)
+    POTAMD_ROW_CASE(256, 2, 1)
+    POTAMD_ROW_CASE(256, 4, 1)
+    POTAMD_ROW_CASE(512, 4, 1)
+    POTAMD_ROW_CASE(1024, 4, 1)
+    POTAMD_ROW_CASE(1024, 8, 1)
   }
 #undef POTAMD_ROW_CASE
   return hipErrorInvalidConfiguration;
@@ -525,39 +544,52 @@ hipError_t launch_row_v(int T, int cpt, int domain, int mode, const RowSweepArgs
 
 }  // namespace
 
-bool row_config_valid(int dtype, int T, int cpt) {
-  static const int f32[][2] = {{64, 4}, {256, 4}, {256, 8}, {512, 8}, {1024, 8}, {1024, 16}};
-  static const int f64[][2] = {{64, 2}, {256, 2}, {256, 4}, {512, 4}, {1024, 4}, {1024, 8}};
-  const auto& tab = dtype == 0 ? f32 : f64;
-  for (const auto& e : tab)
-    if (e[0] == T && e[1] == cpt) return true;
+namespace {
+struct RowCfg {
+  int T, cpt, pf;
+};
+// Tile shapes compiled per dtype; the default for m is the first entry that
+// covers it (the autotuner may pick any entry that covers m).
+const RowCfg kRowF32[] = {{64, 4, 1},    {256, 4, 1},   {256, 8, 1},  {512, 8, 1},
+                          {1024, 8, 1},  {1024, 16, 1}, {1024, 8, 2}, {1024, 16, 2},
+                          {512, 32, 1},  {512, 32, 2}};
+const RowCfg kRowF64[] = {{64, 2, 1},  {256, 2, 1},  {256, 4, 1},
+                          {512, 4, 1}, {1024, 4, 1}, {1024, 8, 1}};
+}  // namespace
+
+bool row_config_valid(int dtype, int T, int cpt, int pf) {
+  const RowCfg* tab = dtype == 0 ? kRowF32 : kRowF64;
+  const int cnt = dtype == 0 ? sizeof(kRowF32) / sizeof(RowCfg) : sizeof(kRowF64) / sizeof(RowCfg);
+  for (int q = 0; q < cnt; ++q)
+    if (tab[q].T == T && tab[q].cpt == cpt && tab[q].pf == pf) return true;
   return false;
 }
 
-void row_config_default(int dtype, int m, int* T, int* cpt) {
-  static const int f32[][2] = {{64, 4}, {256, 4}, {256, 8}, {512, 8}, {1024, 8}, {1024, 16}};
-  static const int f64[][2] = {{64, 2}, {256, 2}, {256, 4}, {512, 4}, {1024, 4}, {1024, 8}};
-  const auto& tab = dtype == 0 ? f32 : f64;
+void row_config_default(int dtype, int m, int* T, int* cpt, int* pf) {
+  const RowCfg* tab = dtype == 0 ? kRowF32 : kRowF64;
+  const int cnt = dtype == 0 ? sizeof(kRowF32) / sizeof(RowCfg) : sizeof(kRowF64) / sizeof(RowCfg);
   *T = 0;
   *cpt = 0;
-  for (const auto& e : tab) {
-    if ((int64_t)e[0] * e[1] >= m) {
-      *T = e[0];
-      *cpt = e[1];
+  *pf = 1;
+  for (int q = 0; q < cnt; ++q) {
+    if ((int64_t)tab[q].T * tab[q].cpt >= m) {
+      *T = tab[q].T;
+      *cpt = tab[q].cpt;
+      *pf = tab[q].pf;
       return;
     }
   }
 }
 
-hipError_t launch_row_sweep(int dtype, int vec_ok, int T, int cpt, int domain, int mode,
+hipError_t launch_row_sweep(int dtype, int vec_ok, int T, int cpt, int pf, int domain, int mode,
                             const RowSweepArgs& a, int G, int B, hipStream_t s) {
   dim3 grid(G, B);
   if (dtype == 0) {
-    return vec_ok ? launch_row_v<float, 4>(T, cpt, domain, mode, a, grid, s)
-                  : launch_row_v<float, 1>(T, cpt, domain, mode, a, grid, s);
+    return vec_ok ? launch_row_v<float, 4>(T, cpt, pf, domain, mode, a, grid, s)
+                  : launch_row_v<float, 1>(T, cpt, pf, domain, mode, a, grid, s);
   }
-  return vec_ok ? launch_row_v<double, 2>(T, cpt, domain, mode, a, grid, s)
-                : launch_row_v<double, 1>(T, cpt, domain, mode, a, grid, s);
+  return vec_ok ? launch_row_v<double, 2>(T, cpt, pf, domain, mode, a, grid, s)
+                : launch_row_v<double, 1>(T, cpt, pf, domain, mode, a, grid, s);
 }
 
 hipError_t launch_col_merge(int dtype, const MergeArgs& a, int B, hipStream_t s) {

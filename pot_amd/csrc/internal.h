@@ -96,6 +96,7 @@ struct BatchArgs {
   double* v;             // out: log_v
   const int* order;      // [B] launch order (largest first) or null
   int max_iter;
+  int check_every;       // iterations between marginal checks (10 in the reference)
   double stop_thr;
   double fallback_thr;   // > 0: restart cold when the first check exceeds it
   int* niter;            // [B]
@@ -106,7 +107,7 @@ struct BatchArgs {
 
 int batch_cpt(int max_m);
 int batch_waves_default();
-hipError_t launch_batch_log(int cpt, int vec_ok, int waves, const BatchArgs& a, int B,
+hipError_t launch_batch_log(int cpt, int vec_ok, int waves, int pf, const BatchArgs& a, int B,
                             hipStream_t s);
 
 struct RaggedDistArgs {
@@ -161,6 +162,7 @@ enum MlpOpCode : int {
   kOpOuterAcc = 11,
   kOpAdam = 12,
   kOpToBf16 = 13,
+  kOpRowNorm = 14,
 };
 
 struct MlpOp {
@@ -184,9 +186,9 @@ int plan_blocks(int n);
 hipError_t launch_plan(int dtype, PlanArgs a, int B, double* sums, hipStream_t s);
 hipError_t launch_sqeuclid(int dtype, const DistArgs& a, int B, hipStream_t s);
 
-bool row_config_valid(int dtype, int T, int cpt);
-void row_config_default(int dtype, int m, int* T, int* cpt);
-hipError_t launch_row_sweep(int dtype, int vec_ok, int T, int cpt, int domain, int mode,
+bool row_config_valid(int dtype, int T, int cpt, int pf);
+void row_config_default(int dtype, int m, int* T, int* cpt, int* pf);
+hipError_t launch_row_sweep(int dtype, int vec_ok, int T, int cpt, int pf, int domain, int mode,
                             const RowSweepArgs& a, int G, int B, hipStream_t s);
 hipError_t launch_col_merge(int dtype, const MergeArgs& a, int B, hipStream_t s);
 hipError_t launch_col_lse(int dtype, const ColLseArgs& a, int G, int B, const double* logb2,

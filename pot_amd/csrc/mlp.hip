@@ -27,6 +27,14 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 
 constexpr int kTM = 64, kTN = 64, kTK = 32, kPad = 8;
+// Activation of every hidden layer: leaky ReLU (no unit can die).  A stored
+// activation has the sign of its pre-activation, so backward masks read it.
+constexpr float kLeak = 0.01f;
+// Input features: the weight column enters as seg_len * w - 1 (0 for uniform
+// weights) and log reg as (log reg - log 0.0158) / 1.1513, i.e. [-1, 1] over
+// reg in [5e-3, 5e-2].
+constexpr double kLogRegCenter = -4.147366;
+constexpr double kLogRegScale = 1.151293;
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   __shared__ uint16_t As[kTM][kTK + kPad];
@@ -81,8 +89,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
       if (i >= g.M) continue;
       float x = acc[4 * q + r] * g.alpha + bj;
       if (g.rowvec) x += g.rowvec[(int64_t)(i / g.seg_len) * g.N + j];
-      if (g.relu) x = x > 0.f ? x : 0.f;
-      if (g.mask) x = bf2f(static_cast<const uint16_t*>(g.mask)[(int64_t)i * g.ldmask + j]) > 0.f ? x : 0.f;
+      if (g.relu) x = x > 0.f ? x : kLeak * x;
+      if (g.mask) x = bf2f(static_cast<const uint16_t*>(g.mask)[(int64_t)i * g.ldmask + j]) > 0.f ? x : kLeak * x;
       const int64_t o = (int64_t)i * g.ldc + j;
       if (g.out_kind == 0) {
         static_cast<uint16_t*>(g.C)[o] = f2bf(x);
@@ -116,9 +124,9 @@ __global__ void build_inputs_kernel(int rows, int seg_len, int dx, int width, co
   if (c < dx) {
     v = pts[(int64_t)r * dx + c] * scale;
   } else if (with_extra && c == dx) {
-    v = (float)(b[r] * seg_len);
+    v = (float)(b[r] * seg_len - 1.0);
   } else if (with_extra && c == dx + 1) {
-    v = (float)log(regs[p]);
+    v = (float)((log(regs[p]) - kLogRegCenter) / kLogRegScale);
   }
   out[e] = f2bf(v);
 }
@@ -133,13 +141,14 @@ __global__ void segment_mean_kernel(int seg_len, int h, const uint16_t* H, float
   }
 }
 
-// out[p][n] = sum_k in[p][k] W[k][n] (+ bias[n]) -- small fp32 products with
+// out[p][n] = sum_k in[p][k] W[k][n] (+ bias[n], or bias[p][n] when bias_per_row) --
+// small fp32 products with
 // the per-problem vectors (W fp32, row-major, k x n).
 __global__ void vec_mat_kernel(int P, int k, int n, const float* in, const float* W,
-                               const float* bias, float* out) {
+                               const float* bias, int bias_per_row, float* out) {
   const int p = blockIdx.x;
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    float s = bias ? bias[c] : 0.f;
+    float s = bias ? bias[(bias_per_row ? (int64_t)p * n : 0) + c] : 0.f;
     for (int q = 0; q < k; ++q) s = fmaf(in[(int64_t)p * k + q], W[(int64_t)q * n + c], s);
     out[(int64_t)p * n + c] = s;
   }
@@ -155,16 +164,31 @@ __global__ void vec_matT_kernel(int P, int k, int n, const float* in, const floa
   }
 }
 
-// f[r] = sum_h G[r][h] w[h] + b0 (one wave per row).
+// f[r] = sum_h G[r][h] w[h] + b0 (+ qn[r]) (one wave per row).
 __global__ void head_kernel(int rows, int h, const uint16_t* G, const float* w, const float* b0,
-                            float* f) {
+                            const float* qn, float* f) {
   const int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (r >= rows) return;
   float s = 0.f;
   for (int c = lane; c < h; c += 64) s = fmaf(bf2f(G[(int64_t)r * h + c]), w[c], s);
   s = wave_sum(s);
-  if (lane == 0) f[r] = s + b0[0];
+  if (lane == 0) f[r] = s + b0[0] + (qn ? qn[r] : 0.f);
+}
+
+// qn[r] = |pts[r] * scale_p|^2 over the first dx coordinates (fp32).
+__global__ void row_norm_kernel(int rows, int seg_len, int dx, const float* pts,
+                                const float* cmax, float* qn) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const int p = r / seg_len;
+  const float sc2 = 1.f / (cmax ? cmax[p] : 1.f);
+  float s = 0.f;
+  for (int c = 0; c < dx; ++c) {
+    const float v = pts[(int64_t)r * dx + c];
+    s = fmaf(v, v, s);
+  }
+  qn[r] = s * sc2;
 }
 
 // Centred weighted MSE per problem and its gradient with respect to f.
@@ -207,13 +231,13 @@ __global__ void loss_kernel(int seg_len, int P, const float* f, const float* t,
   if (tid == 0 && loss_out) loss_out[p] = L / seg_len * wgt / P;
 }
 
-// dB[r][h] = df[r] * w[h] * (G[r][h] > 0)  (bf16 out)
+// dB[r][h] = df[r] * w[h] * leak'(G[r][h])  (bf16 out)
 __global__ void head_back_kernel(int rows, int h, const float* df, const float* w,
                                  const uint16_t* G, uint16_t* dB) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)rows * h) return;
   const int r = (int)(e / h), c = (int)(e % h);
-  dB[e] = f2bf(bf2f(G[e]) > 0.f ? df[r] * w[c] : 0.f);
+  dB[e] = f2bf(df[r] * w[c] * (bf2f(G[e]) > 0.f ? 1.f : kLeak));
 }
 
 // gw[h] += sum_r G[r][h] df[r] ; gb += sum_r df[r]   (atomic per block)
@@ -249,14 +273,14 @@ __global__ void segment_sum_kernel(int seg_len, int h, const uint16_t* X, float*
   }
 }
 
-// dA[r][c] = dz[seg(r)][c] / seg_len * (H[r][c] > 0)   (bf16 out)
+// dA[r][c] = dz[seg(r)][c] / seg_len * leak'(H[r][c])   (bf16 out)
 __global__ void pool_back_kernel(int rows, int seg_len, int h, const float* dz, const uint16_t* H,
                                  uint16_t* dA) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)rows * h) return;
   const int r = (int)(e / h), c = (int)(e % h);
   const int p = r / seg_len;
-  dA[e] = f2bf(bf2f(H[e]) > 0.f ? dz[(int64_t)p * h + c] / seg_len : 0.f);
+  dA[e] = f2bf(dz[(int64_t)p * h + c] / seg_len * (bf2f(H[e]) > 0.f ? 1.f : kLeak));
 }
 
 // gW[q][c] += sum_p z[p][q] dc[p][c]  (outer products of per-problem vectors)
@@ -307,7 +331,7 @@ hipError_t launch_mlp_op(const MlpOp& o, hipStream_t s) {
     case kOpVecMat:
       hipLaunchKernelGGL(vec_mat_kernel, dim3(o.P), dim3(256), 0, s, o.P, o.k, o.n,
                          static_cast<const float*>(o.x), static_cast<const float*>(o.w),
-                         static_cast<const float*>(o.y), static_cast<float*>(o.out));
+                         static_cast<const float*>(o.y), o.flag, static_cast<float*>(o.out));
       break;
     case kOpVecMatT:
       hipLaunchKernelGGL(vec_matT_kernel, dim3(o.P), dim3(256), 0, s, o.P, o.k, o.n,
@@ -317,7 +341,8 @@ hipError_t launch_mlp_op(const MlpOp& o, hipStream_t s) {
     case kOpHead:
       hipLaunchKernelGGL(head_kernel, dim3(nb(o.rows, 4)), dim3(256), 0, s, o.rows, o.k,
                          static_cast<const uint16_t*>(o.x), static_cast<const float*>(o.w),
-                         static_cast<const float*>(o.y), static_cast<float*>(o.out));
+                         static_cast<const float*>(o.y), static_cast<const float*>(o.z),
+                         static_cast<float*>(o.out));
       break;
     case kOpLoss:
       hipLaunchKernelGGL(loss_kernel, dim3(o.P), dim3(256), 0, s, o.seg_len, o.P,
@@ -361,6 +386,11 @@ hipError_t launch_mlp_op(const MlpOp& o, hipStream_t s) {
                          static_cast<float*>(const_cast<void*>(o.z)), o.hp[0], o.hp[1],
                          o.hp[2], o.hp[3], o.hp[4], o.hp[5], o.hp[6],
                          static_cast<uint16_t*>(o.out2));
+      break;
+    case kOpRowNorm:
+      hipLaunchKernelGGL(row_norm_kernel, dim3(nb(o.rows, 256)), dim3(256), 0, s, o.rows,
+                         o.seg_len, o.n, static_cast<const float*>(o.x),
+                         static_cast<const float*>(o.w), static_cast<float*>(o.out));
       break;
     case kOpToBf16:
       hipLaunchKernelGGL(to_bf16_kernel, dim3(nb(o.count, 256)), dim3(256), 0, s, o.count,

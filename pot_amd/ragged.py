@@ -105,8 +105,12 @@ def from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None):
     """
     device = device or _device.require_gpu()
     lib = _lib.load()
-    ns = [len(x) for x in xs]
-    ms = [len(y) for y in ys]
+    if isinstance(xs, torch.Tensor) and isinstance(ys, torch.Tensor):
+        ns = [int(xs.shape[1])] * int(xs.shape[0])
+        ms = [int(ys.shape[1])] * int(ys.shape[0])
+    else:
+        ns = [len(x) for x in xs]
+        ms = [len(y) for y in ys]
     d = int(xs[0].shape[1])
     ns, ms, total, moff, ns_d, ms_d, ld_d, aoff, boff = _layout(ns, ms, device)
     X = _pack_points(xs, device)
@@ -175,12 +179,14 @@ def launch_order(R, work=None):
 
 
 def sinkhorn_log_ragged(R, u0=None, numItermax=1000, stopThr=1e-9, order=None,
-                        fallback_thr=0.0, waves=0):
+                        fallback_thr=0.0, waves=0, prefetch=0, check_every=10):
     """Solve every problem of ``R`` with sinkhorn_log semantics in one launch.
 
     u0: (sum n,) float64 initial log potentials (None = zeros, the cold start).
     fallback_thr > 0: a problem whose first checked error exceeds it restarts
-    cold (its status gets bit 4).  Returns a RaggedResult; the call is
+    cold (its status gets bit 4).  check_every: iterations between checks (10
+    is the reference's cadence and keeps ``niter`` identical to a lone
+    sinkhorn_log call; 1 stops at the first iteration meeting stopThr).  Returns a RaggedResult; the call is
     asynchronous on the current stream.
     """
     lib = _lib.load()
@@ -199,8 +205,10 @@ def sinkhorn_log_ragged(R, u0=None, numItermax=1000, stopThr=1e-9, order=None,
     rc = lib.potamd_sinkhorn_log_batch(
         _lib.F32, B, int(R.ns.max()), int(R.ms.max()), _ptr(R.M), _ptr(R.moff), _ptr(R.ns_d),
         _ptr(R.ms_d), _ptr(R.ldms_d), _ptr(R.aoff), _ptr(R.boff), _ptr(R.regs), _ptr(R.a),
-        _ptr(R.b), _ptr(u), _ptr(v), int(numItermax), float(stopThr), float(fallback_thr),
-        _ptr(order), 1, _ptr(niter), _ptr(status), _ptr(e_last), _ptr(e_first), int(waves),
+        _ptr(R.b), _ptr(u), _ptr(v), int(numItermax), int(check_every), float(stopThr),
+        float(fallback_thr),
+        _ptr(order), 1, _ptr(niter), _ptr(status), _ptr(e_last), _ptr(e_first),
+        int(waves) | (int(prefetch) << 8),
         ctypes.c_void_p(_device.stream_ptr(dev)))
     _lib.check(rc, "potamd_sinkhorn_log_batch")
     return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,

@@ -10,7 +10,7 @@ import os
 
 # A warm-started problem whose first checked marginal error exceeds this
 # restarts from the cold start.
-FALLBACK_THR = 1e-2
+FALLBACK_THR = 1e-1
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights.npz")
 

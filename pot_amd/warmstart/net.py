@@ -1,16 +1,23 @@
-"""Warm-start network on the GPU: DeepSets encoder + per-point MLP.
+"""Warm-start network on the GPU: DeepSets encoders + per-point MLP.
 
-Per problem (sources x: n x 16, targets y: m x 16, target weights b, reg),
-with points scaled by 1/sqrt(max cost) so every problem is scale free:
+Per problem (sources x: n x 16 with weights a, targets y: m x 16 with
+weights b, reg), with points scaled by 1/sqrt(max cost) so every problem is
+scale free:
 
-    e_j = [y_j, m b_j, log reg]      H1 = relu(e W_e1 + b_e1)
-    H2  = relu(H1 W_e2 + b_e2)       z  = mean_j H2
-    c   = z W_m1z + b_m1             G1 = relu(x W_m1x + c)
-    G2  = relu(G1 W_m2 + b_m2)       f  = G2 w_m3 + b_m3
+    e_j = [y_j, m b_j - 1, lr]       H1 = act(e W_e1 + b_e1)
+    H2  = act(H1 W_e2 + b_e2)        zt = mean_j H2      (target encoder)
+    s_i = [x_i, n a_i - 1, lr]       S1 = act(s W_s1 + b_s1)
+    S2  = act(S1 W_s2 + b_s2)        zs = mean_i S2      (source encoder)
+    c   = zt W_m1z + zs W_m1s + b_m1 G1 = act(s W_m1x + c)
+    G2  = act(G1 W_m2 + b_m2)        f  = |x_i|^2 + G2 w_m3 + b_m3
 
-``[x_i, z] W_m1`` is split into ``x_i W_m1x + z W_m1z`` so the z term is
-computed once per problem.  f is the dual potential in units of the
-normalised cost; the solver starts from u0 = f / reg.
+with act the leaky ReLU (slope 0.01) and lr = (log reg + 4.147) / 1.151.
+
+``[s_i, zt, zs] W_m1`` is split so the pooled terms are computed once per
+problem.  The exact |x_i|^2 term leaves the network the c-transform part of
+the potential (a soft-max of functions affine in x).  f is the dual
+potential in units of the normalised cost; the solver starts from
+u0 = f / reg.
 
 Every forward/backward step runs in libpot_amd.so: the four wide layers on
 the bf16 MFMA GEMM (fp32 accumulation, fused bias / per-problem broadcast /
@@ -33,14 +40,16 @@ KIN = 32  # input width padded to one GEMM K slice
 # op codes of potamd_mlp_op
 OP_BUILD, OP_SEGMEAN, OP_VECMAT, OP_VECMATT, OP_HEAD, OP_LOSS = 0, 1, 2, 3, 4, 5
 OP_HEADBACK, OP_HEADWGRAD, OP_COLSUM, OP_SEGSUM, OP_POOLBACK, OP_OUTER = 6, 7, 8, 9, 10, 11
-OP_ADAM, OP_TOBF16 = 12, 13
+OP_ADAM, OP_TOBF16, OP_ROWNORM = 12, 13, 14
 
 PARAM_SHAPES = {
     "W_e1": (KIN, H), "b_e1": (H,), "W_e2": (H, H), "b_e2": (H,),
-    "W_m1x": (KIN, H), "W_m1z": (H, H), "b_m1": (H,), "W_m2": (H, H), "b_m2": (H,),
-    "w_m3": (H,), "b_m3": (1,),
+    "W_s1": (KIN, H), "b_s1": (H,), "W_s2": (H, H), "b_s2": (H,),
+    "W_m1x": (KIN, H), "W_m1z": (H, H), "W_m1s": (H, H), "b_m1": (H,),
+    "W_m2": (H, H), "b_m2": (H,), "w_m3": (H,), "b_m3": (1,),
 }
-GEMM_WEIGHTS = ("W_e1", "W_e2", "W_m1x", "W_m2")
+GEMM_WEIGHTS = ("W_e1", "W_e2", "W_s1", "W_s2", "W_m1x", "W_m2")
+PADDED_INPUT = ("W_e1", "W_s1", "W_m1x")  # rows >= DX + 2 multiply zero padding
 R0 = 1e-2
 
 
@@ -73,10 +82,13 @@ class WarmStartNet:
         p = {k: np.zeros(s) for k, s in PARAM_SHAPES.items()}
         p["W_e1"][:DX + 2] = lin(DX + 2, DX + 2, H)
         p["W_e2"] = lin(H, H, H)
-        p["W_m1x"][:DX] = lin(DX + H, DX, H)
-        p["W_m1z"] = lin(DX + H, H, H)
+        p["W_s1"][:DX + 2] = lin(DX + 2, DX + 2, H)
+        p["W_s2"] = lin(H, H, H)
+        p["W_m1x"][:DX + 2] = lin(DX + 2 + 2 * H, DX + 2, H)
+        p["W_m1z"] = lin(DX + 2 + 2 * H, H, H)
+        p["W_m1s"] = lin(DX + 2 + 2 * H, H, H)
         p["W_m2"] = lin(H, H, H)
-        p["w_m3"] = lin(H, H, 1)[:, 0] * 0.1
+        # zero head: training starts from the exact |x|^2 predictor
         return p
 
     @classmethod
@@ -121,34 +133,50 @@ class WarmStartNet:
             self._op(OP_TOBF16, count=self.w[k].numel(), x=self.w[k], out=self.wb[k])
 
     # ------------------------------------------------------------- inputs
-    def build_inputs(self, pts, P, seg_len, b=None, regs=None, cmax=None, extra=False):
-        """(P*seg_len, KIN) bf16 rows: scaled points (+ [m b, log reg] if extra)."""
+    def build_inputs(self, pts, P, seg_len, w, regs, cmax):
+        """(P*seg_len, KIN) bf16 rows [scaled point, seg_len * weight, log reg, 0...]."""
         rows = P * seg_len
         out = torch.empty((rows, KIN), dtype=torch.bfloat16, device=self.device)
-        self._op(OP_BUILD, rows=rows, seg_len=seg_len, n=DX, k=KIN, flag=int(extra),
-                 x=pts.contiguous(), y=b, z=regs, w=cmax, out=out)
+        self._op(OP_BUILD, rows=rows, seg_len=seg_len, n=DX, k=KIN, flag=1,
+                 x=pts.contiguous(), y=w, z=regs, w=cmax, out=out)
+        return out
+
+    def row_norms(self, pts, P, seg_len, cmax):
+        """(P*seg_len,) fp32 squared norms of the scaled points."""
+        out = torch.empty(P * seg_len, dtype=torch.float32, device=self.device)
+        self._op(OP_ROWNORM, rows=P * seg_len, seg_len=seg_len, n=DX, x=pts.contiguous(),
+                 w=cmax, out=out)
         return out
 
     # ------------------------------------------------------------ forward
-    def forward(self, Xin, Ein, P, n, m, keep=False):
+    def _encode(self, inp, rows, P, seg, W1, b1, W2, b2):
+        dev = self.device
+        A = torch.empty((rows, H), dtype=torch.bfloat16, device=dev)
+        self._gemm(inp, self.wb[W1], A, rows, H, KIN, bias=self.w[b1], relu=1)
+        B = torch.empty_like(A)
+        self._gemm(A, self.wb[W2], B, rows, H, H, bias=self.w[b2], relu=1)
+        z = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._op(OP_SEGMEAN, seg_len=seg, P=P, k=H, x=B, out=z)
+        return A, B, z
+
+    def forward(self, Sin, Ein, qn, P, n, m, keep=False):
+        """f (P*n,) from source rows Sin, target rows Ein and source norms qn."""
         dev = self.device
         w, wb = self.w, self.wb
-        H1 = torch.empty((P * m, H), dtype=torch.bfloat16, device=dev)
-        self._gemm(Ein, wb["W_e1"], H1, P * m, H, KIN, bias=w["b_e1"], relu=1)
-        H2 = torch.empty_like(H1)
-        self._gemm(H1, wb["W_e2"], H2, P * m, H, H, bias=w["b_e2"], relu=1)
-        z = torch.empty((P, H), dtype=torch.float32, device=dev)
-        self._op(OP_SEGMEAN, seg_len=m, P=P, k=H, x=H2, out=z)
+        H1, H2, zt = self._encode(Ein, P * m, P, m, "W_e1", "b_e1", "W_e2", "b_e2")
+        S1, S2, zs = self._encode(Sin, P * n, P, n, "W_s1", "b_s1", "W_s2", "b_s2")
         c = torch.empty((P, H), dtype=torch.float32, device=dev)
-        self._op(OP_VECMAT, P=P, k=H, n=H, x=z, w=w["W_m1z"], y=w["b_m1"], out=c)
+        self._op(OP_VECMAT, P=P, k=H, n=H, x=zt, w=w["W_m1z"], y=w["b_m1"], out=c)
+        c2 = torch.empty_like(c)
+        self._op(OP_VECMAT, P=P, k=H, n=H, flag=1, x=zs, w=w["W_m1s"], y=c, out=c2)
         G1 = torch.empty((P * n, H), dtype=torch.bfloat16, device=dev)
-        self._gemm(Xin, wb["W_m1x"], G1, P * n, H, KIN, rowvec=c, seg_len=n, relu=1)
+        self._gemm(Sin, wb["W_m1x"], G1, P * n, H, KIN, rowvec=c2, seg_len=n, relu=1)
         G2 = torch.empty_like(G1)
         self._gemm(G1, wb["W_m2"], G2, P * n, H, H, bias=w["b_m2"], relu=1)
         f = torch.empty(P * n, dtype=torch.float32, device=dev)
-        self._op(OP_HEAD, rows=P * n, k=H, x=G2, w=w["w_m3"], y=w["b_m3"], out=f)
+        self._op(OP_HEAD, rows=P * n, k=H, x=G2, w=w["w_m3"], y=w["b_m3"], z=qn, out=f)
         if keep:
-            self._cache = (Xin, Ein, P, n, m, H1, H2, z, G1, G2)
+            self._cache = (Sin, Ein, P, n, m, H1, H2, zt, S1, S2, zs, G1, G2)
         return f
 
     # ----------------------------------------------------------- backward
@@ -159,8 +187,20 @@ class WarmStartNet:
         self._op(OP_LOSS, seg_len=n, P=P, scalar=R0, x=f, y=target, z=regs, out=df, out2=lp)
         return lp, df
 
+    def _encode_back(self, inp, A, B, dz, rows, seg, W1, b1, W2, b2):
+        dev, g, wb = self.device, self.g, self.wb
+        dA2 = torch.empty((rows, H), dtype=torch.bfloat16, device=dev)
+        self._op(OP_POOLBACK, rows=rows, seg_len=seg, k=H, x=dz, y=B, out=dA2)
+        splits = max(1, min(64, rows // 512))
+        self._gemm(A, dA2, g[W2], H, H, rows, transA=1, out_kind=2, splits=splits)
+        self._op(OP_COLSUM, rows=rows, k=H, x=dA2, out=g[b2])
+        dA1 = torch.empty_like(dA2)
+        self._gemm(dA2, wb[W2], dA1, rows, H, H, transB=1, mask=A)
+        self._gemm(inp, dA1, g[W1], KIN, H, rows, transA=1, out_kind=2, splits=splits)
+        self._op(OP_COLSUM, rows=rows, k=H, x=dA1, out=g[b1])
+
     def backward(self, df):
-        Xin, Ein, P, n, m, H1, H2, z, G1, G2 = self._cache
+        Sin, Ein, P, n, m, H1, H2, zt, S1, S2, zs, G1, G2 = self._cache
         dev = self.device
         w, wb, g = self.w, self.wb, self.g
         for t in g.values():
@@ -174,23 +214,19 @@ class WarmStartNet:
         self._op(OP_COLSUM, rows=rows_n, k=H, x=dB2, out=g["b_m2"])
         dB1 = torch.empty_like(dB2)
         self._gemm(dB2, wb["W_m2"], dB1, rows_n, H, H, transB=1, mask=G1)
-        self._gemm(Xin, dB1, g["W_m1x"], KIN, H, rows_n, transA=1, out_kind=2, splits=splits)
+        self._gemm(Sin, dB1, g["W_m1x"], KIN, H, rows_n, transA=1, out_kind=2, splits=splits)
         dc = torch.empty((P, H), dtype=torch.float32, device=dev)
         self._op(OP_SEGSUM, seg_len=n, P=P, k=H, x=dB1, out=dc)
         ones = torch.ones((P, 1), dtype=torch.float32, device=dev)
         self._op(OP_OUTER, P=P, k=1, n=H, x=ones, y=dc, out=g["b_m1"])
-        self._op(OP_OUTER, P=P, k=H, n=H, x=z, y=dc, out=g["W_m1z"])
-        dz = torch.empty((P, H), dtype=torch.float32, device=dev)
-        self._op(OP_VECMATT, P=P, k=H, n=H, x=dc, w=w["W_m1z"], out=dz)
-        dA2 = torch.empty((rows_m, H), dtype=torch.bfloat16, device=dev)
-        self._op(OP_POOLBACK, rows=rows_m, seg_len=m, k=H, x=dz, y=H2, out=dA2)
-        splits_m = max(1, min(64, rows_m // 512))
-        self._gemm(H1, dA2, g["W_e2"], H, H, rows_m, transA=1, out_kind=2, splits=splits_m)
-        self._op(OP_COLSUM, rows=rows_m, k=H, x=dA2, out=g["b_e2"])
-        dA1 = torch.empty_like(dA2)
-        self._gemm(dA2, wb["W_e2"], dA1, rows_m, H, H, transB=1, mask=H1)
-        self._gemm(Ein, dA1, g["W_e1"], KIN, H, rows_m, transA=1, out_kind=2, splits=splits_m)
-        self._op(OP_COLSUM, rows=rows_m, k=H, x=dA1, out=g["b_e1"])
+        self._op(OP_OUTER, P=P, k=H, n=H, x=zt, y=dc, out=g["W_m1z"])
+        self._op(OP_OUTER, P=P, k=H, n=H, x=zs, y=dc, out=g["W_m1s"])
+        dzt = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._op(OP_VECMATT, P=P, k=H, n=H, x=dc, w=w["W_m1z"], out=dzt)
+        dzs = torch.empty_like(dzt)
+        self._op(OP_VECMATT, P=P, k=H, n=H, x=dc, w=w["W_m1s"], out=dzs)
+        self._encode_back(Ein, H1, H2, dzt, rows_m, m, "W_e1", "b_e1", "W_e2", "b_e2")
+        self._encode_back(Sin, S1, S2, dzs, rows_n, n, "W_s1", "b_s1", "W_s2", "b_s2")
         return g
 
     def adam_step(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
@@ -202,17 +238,17 @@ class WarmStartNet:
             self._op(OP_ADAM, count=wt.numel(), hp=(lr, b1, b2, eps, bc1, bc2, wd),
                      x=self.g[k], y=self.m1[k], z=self.m2[k], out=wt, out2=out2)
         # padded input rows of the first layers stay exactly zero
-        self.w["W_e1"][DX + 2:].zero_()
-        self.w["W_m1x"][DX:].zero_()
-        self.wb["W_e1"][DX + 2:].zero_()
-        self.wb["W_m1x"][DX:].zero_()
+        for k in PADDED_INPUT:
+            self.w[k][DX + 2:].zero_()
+            self.wb[k][DX + 2:].zero_()
 
     # ---------------------------------------------------------- inference
-    def predict_f(self, X, Y, b, regs, cmax, P, n, m):
-        """f (P*n,) float32 for uniform-size problems; X: (P*n, 16) fp32."""
-        Xin = self.build_inputs(X, P, n, cmax=cmax)
-        Ein = self.build_inputs(Y, P, m, b=b, regs=regs, cmax=cmax, extra=True)
-        return self.forward(Xin, Ein, P, n, m)
+    def inputs(self, X, Y, a, b, regs, cmax, P, n, m):
+        """Network inputs of P equal-size problems (X: (P*n, 16) fp32, ...)."""
+        Sin = self.build_inputs(X, P, n, a, regs, cmax)
+        Ein = self.build_inputs(Y, P, m, b, regs, cmax)
+        qn = self.row_norms(X, P, n, cmax)
+        return Sin, Ein, qn
 
     def predict_u0(self, R, X, Y, regs):
         """Initial log potentials u0 = f / reg for the ragged batch R (equal sizes)."""
@@ -221,6 +257,7 @@ class WarmStartNet:
         if not (np.all(R.ns == n) and np.all(R.ms == m)):
             raise ValueError("the warm-start network expects problems of one size")
         regs_t = torch.as_tensor(np.asarray(regs, dtype=np.float64)).to(self.device)
-        f = self.predict_f(X.reshape(-1, DX).float(), Y.reshape(-1, DX).float(), R.b, regs_t,
-                           R.cmax, P, n, m)
+        Sin, Ein, qn = self.inputs(X.reshape(-1, DX).float(), Y.reshape(-1, DX).float(), R.a,
+                                   R.b, regs_t, R.cmax, P, n, m)
+        f = self.forward(Sin, Ein, qn, P, n, m)
         return (f.double().view(P, n) / regs_t[:, None]).reshape(-1)

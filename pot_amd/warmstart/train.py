@@ -53,7 +53,7 @@ def make_dataset(net, count, seed, chunk=2048, label_tol=1e-6, max_iter=5000, lo
     dev = net.device
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
-    Xs, Es, Ts, Rs = [], [], [], []
+    Xs, Es, Qs, Ts, Rs = [], [], [], [], []
     done, t0, iters = 0, time.time(), []
     while done < count:
         P = min(chunk, count - done)
@@ -62,20 +62,22 @@ def make_dataset(net, count, seed, chunk=2048, label_tol=1e-6, max_iter=5000, lo
         res = ragged.sinkhorn_log_ragged(R, numItermax=max_iter, stopThr=label_tol)
         n = x.shape[1]
         f = (res.u.view(P, n) * regs[:, None]).float()
-        Xs.append(net.build_inputs(x.view(-1, DX), P, n, cmax=R.cmax).view(P, n, -1))
-        Es.append(net.build_inputs(y.view(-1, DX), P, n, b=R.b, regs=regs, cmax=R.cmax,
-                                   extra=True).view(P, n, -1))
+        Sin, Ein, qn = net.inputs(x.view(-1, DX), y.view(-1, DX), R.a, R.b, regs, R.cmax,
+                                  P, n, n)
+        Xs.append(Sin.view(P, n, -1))
+        Es.append(Ein.view(P, n, -1))
+        Qs.append(qn.view(P, n))
         Ts.append(f)
         Rs.append(regs)
         iters.append(res.niter.float().mean().item())
         done += P
         log(f"  labels {done}/{count}  mean iters {iters[-1]:.1f}  "
             f"{time.time() - t0:.1f}s")
-    return torch.cat(Xs), torch.cat(Es), torch.cat(Ts), torch.cat(Rs)
+    return torch.cat(Xs), torch.cat(Es), torch.cat(Qs), torch.cat(Ts), torch.cat(Rs)
 
 
 def train(net, data, epochs, batch=32, lr=1e-3, lr_min=1e-5, val=None, log=print):
-    X, E, T, Rg = data
+    X, E, Q, T, Rg = data
     N, n = T.shape
     m = E.shape[1]
     steps_per_epoch = N // batch
@@ -92,9 +94,10 @@ def train(net, data, epochs, batch=32, lr=1e-3, lr_min=1e-5, val=None, log=print
             idx = perm[s * batch:(s + 1) * batch]
             xb = X[idx].reshape(batch * n, -1).contiguous()
             eb = E[idx].reshape(batch * m, -1).contiguous()
+            qb = Q[idx].reshape(-1).contiguous()
             tb = T[idx].reshape(-1).contiguous()
             rb = Rg[idx].contiguous()
-            f = net.forward(xb, eb, batch, n, m, keep=True)
+            f = net.forward(xb, eb, qb, batch, n, m, keep=True)
             lp, df = net.loss_grad(f, tb, rb, batch, n)
             net.backward(df)
             cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * step / max(1, total)))
@@ -113,14 +116,15 @@ def train(net, data, epochs, batch=32, lr=1e-3, lr_min=1e-5, val=None, log=print
 
 
 def evaluate(net, data, batch=32):
-    X, E, T, Rg = data
+    X, E, Q, T, Rg = data
     N, n = T.shape
     m = E.shape[1]
     tot = 0.0
     cnt = 0
     for s in range(0, N - batch + 1, batch):
         f = net.forward(X[s:s + batch].reshape(batch * n, -1).contiguous(),
-                        E[s:s + batch].reshape(batch * m, -1).contiguous(), batch, n, m)
+                        E[s:s + batch].reshape(batch * m, -1).contiguous(),
+                        Q[s:s + batch].reshape(-1).contiguous(), batch, n, m)
         lp, _ = net.loss_grad(f, T[s:s + batch].reshape(-1).contiguous(),
                               Rg[s:s + batch].contiguous(), batch, n)
         tot += lp.sum().item()

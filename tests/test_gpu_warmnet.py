@@ -76,80 +76,109 @@ def test_gemm_epilogues_and_split_k():
     assert ((G - ref).abs().max() / ref.abs().max()).item() < 1e-4
 
 
-def _tiny_problem(seed, P=2, n=64, m=64):
-    rng = np.random.default_rng(seed)
-    xs = [rng.standard_normal((n, 16)).astype(np.float32) for _ in range(P)]
-    ys = [rng.standard_normal((m, 16)).astype(np.float32) + 0.5 for _ in range(P)]
-    bs = [np.full(m, 1.0 / m) for _ in range(P)]
-    regs = np.array([0.01, 0.03])[:P]
-    ts = [rng.standard_normal(n) * 0.1 for _ in range(P)]
-    return xs, ys, bs, regs, ts
+def _bf16(x):
+    """Round-to-nearest-even to bfloat16, returned as float64."""
+    f = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32).astype(np.float64)
 
 
 def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
+def _net_inputs(net, P, n, m, seed):
+    rng = np.random.default_rng(seed)
+    dev = net.device
+    X = torch.tensor(rng.standard_normal((P * n, 16)).astype(np.float32), device=dev)
+    Y = torch.tensor(rng.standard_normal((P * m, 16)).astype(np.float32) + 0.5, device=dev)
+    a = torch.full((P * n,), 1.0 / n, dtype=torch.float64, device=dev)
+    b = torch.full((P * m,), 1.0 / m, dtype=torch.float64, device=dev)
+    regs = torch.tensor([0.01, 0.03][:P], dtype=torch.float64, device=dev)
+    cmax = torch.full((P,), 4.0, dtype=torch.float32, device=dev)
+    t = torch.tensor(rng.standard_normal(P * n) * 0.1, dtype=torch.float32, device=dev)
+    return X, Y, a, b, regs, cmax, t
+
+
 def test_forward_backward_match_oracle():
     dev = torch.device("cuda")
-    P, n, m = 2, 64, 64
-    xs, ys, bs, regs, ts = _tiny_problem(1, P, n, m)
+    P, n, m = 2, 64, 96
     params = wnet.WarmStartNet.init_params(3)
     rng = np.random.default_rng(5)
-    for k in ("b_e1", "b_e2", "b_m1", "b_m2"):
+    for k in ("b_e1", "b_e2", "b_s1", "b_s2", "b_m1", "b_m2"):
         params[k] = rng.standard_normal(params[k].shape) * 0.1
+    params["w_m3"] = rng.standard_normal(params["w_m3"].shape) * 0.05
     net = wnet.WarmStartNet(params, dev)
-    X = torch.tensor(np.concatenate(xs), device=dev)
-    Y = torch.tensor(np.concatenate(ys), device=dev)
-    b = torch.tensor(np.concatenate(bs), device=dev)
-    regs_t = torch.tensor(regs, device=dev)
-    cmax = torch.ones(P, dtype=torch.float32, device=dev)
-    Xin = net.build_inputs(X, P, n, cmax=cmax)
-    Ein = net.build_inputs(Y, P, m, b=b, regs=regs_t, cmax=cmax, extra=True)
-    f = net.forward(Xin, Ein, P, n, m, keep=True)
-    target = torch.tensor(np.concatenate(ts), dtype=torch.float32, device=dev)
-    lp, df = net.loss_grad(f, target, regs_t, P, n)
+    X, Y, a, b, regs, cmax, target = _net_inputs(net, P, n, m, 1)
+    Sin, Ein, qn = net.inputs(X, Y, a, b, regs, cmax, P, n, m)
+    f = net.forward(Sin, Ein, qn, P, n, m, keep=True)
+    lp, df = net.loss_grad(f, target, regs, P, n)
     g = net.backward(df)
     torch.cuda.synchronize()
 
-    # oracle on the same bf16-rounded inputs and weights
-    p64 = net.params_numpy()
+    # oracle weights = the GPU's (bf16 copies for the GEMM layers)
+    w = net.params_numpy()
     for k in wnet.GEMM_WEIGHTS:
-        p64[k] = net.wb[k].float().cpu().double().numpy()
-    ox = [Xin.float().cpu().numpy()[p * n:(p + 1) * n, :16].astype(np.float64) for p in range(P)]
-    eo = Ein.float().cpu().numpy().astype(np.float64)
-    oy = [eo[p * m:(p + 1) * m, :16] for p in range(P)]
-    ob = [eo[p * m:(p + 1) * m, 16] / m for p in range(P)]
-    oregs = [float(np.exp(eo[p * m, 17])) for p in range(P)]
-    op = {"W_e1": p64["W_e1"][:18], "b_e1": p64["b_e1"], "W_e2": p64["W_e2"],
-          "b_e2": p64["b_e2"], "W_m1x": p64["W_m1x"][:16], "W_m1z": p64["W_m1z"],
-          "b_m1": p64["b_m1"], "W_m2": p64["W_m2"], "b_m2": p64["b_m2"],
-          "w_m3": p64["w_m3"][:, None], "b_m3": p64["b_m3"]}
-    fo, cache = own.forward(op, ox, oy, ob, oregs)
-    go = own.backward(op, cache, fo, ts, list(regs))
-    lo = own.loss(fo, ts, list(regs))
-    errs = {"f": _rel(f.cpu().numpy(), np.concatenate(fo)),
-            "loss": abs(lp.sum().item() - lo) / lo}
+        w[k] = net.wb[k].float().cpu().double().numpy()
+    op = {k: w[k] for k in own.PARAMS if k not in ("w_m3",)}
+    for k in ("W_e1", "W_s1", "W_m1x"):
+        op[k] = w[k][:18]
+    op["w_m3"] = w["w_m3"][:, None]
+    S = Sin.float().cpu().double().numpy()
+    E = Ein.float().cpu().double().numpy()
+    regs_l = [float(r) for r in regs.cpu().numpy()]
+    xs = [S[p * n:(p + 1) * n, :16] for p in range(P)]
+    ys = [E[p * m:(p + 1) * m, :16] for p in range(P)]
+    as_ = [(S[p * n:(p + 1) * n, 16] + 1.0) / n for p in range(P)]
+    bs = [(E[p * m:(p + 1) * m, 16] + 1.0) / m for p in range(P)]
+    oregs = [float(np.exp(E[p * m, 17] * own.LR_S + own.LR_C)) for p in range(P)]
+    fo, cache = own.forward(op, xs, ys, as_, bs, oregs)
+    ts = list(target.cpu().double().numpy().reshape(P, n))
+    errs = {"f": _rel(f.cpu().numpy(), np.concatenate(fo))}
+    lo = own.loss(fo, ts, regs_l)
+    errs["loss"] = abs(lp.sum().item() - lo) / lo
+    fg = list(f.cpu().double().numpy().reshape(P, n))
+    dfo = own.loss_grad_f(fg, ts, regs_l)
+    errs["df"] = _rel(df.cpu().numpy(), np.concatenate(dfo))
+    assert errs["f"] < 3e-2 and errs["loss"] < 3e-2 and errs["df"] < 1e-4, errs
+
+    # backward from the GPU's own activations and df, with the GPU's bf16
+    # storage of the backward signals emulated: only accumulation order differs
+    Sin_, Ein_, _, _, _, H1, H2, zt, S1, S2, zs, G1, G2 = net._cache
+    t64 = lambda t: t.float().cpu().double().numpy()
+    gc = []
+    for p in range(P):
+        rn, rm = slice(p * n, (p + 1) * n), slice(p * m, (p + 1) * m)
+        te = (t64(H1)[rm], t64(H1)[rm], t64(H2)[rm], t64(H2)[rm], t64(zt)[p])
+        se = (t64(S1)[rn], t64(S1)[rn], t64(S2)[rn], t64(S2)[rn], t64(zs)[p])
+        s_rows = S[rn, :18]
+        gc.append((s_rows[:, :16], E[rm, :18], s_rows, te, se, t64(G1)[rn], t64(G1)[rn],
+                   t64(G2)[rn], t64(G2)[rn]))
+    dfs = [d.astype(np.float64) for d in df.cpu().double().numpy().reshape(P, n)]
+    go = own.backward_from_df(op, gc, dfs, rnd=_bf16)
     for k, ref in go.items():
         mine = g[k].cpu().double().numpy()
-        if k == "W_e1":
+        if k in ("W_e1", "W_s1", "W_m1x"):
             mine = mine[:18]
-        if k == "W_m1x":
-            mine = mine[:16]
         if k == "w_m3":
             ref = ref[:, 0]
+        if k == "b_m3":  # zero by the gauge invariance of the centred loss
+            assert abs(float(mine.sum())) < 1e-6
+            continue
         errs[k] = _rel(mine.reshape(ref.shape), ref)
     print("relative errors:", errs, flush=True)
-    assert all(v < 3e-2 for v in errs.values()), errs
+    assert all(v < 1e-2 for v in errs.values()), errs
 
 
 def test_training_lowers_loss():
+    # overfit a handful of problems: the kernels' gradients must descend
     dev = torch.device("cuda")
     from pot_amd.warmstart import train as wtrain
     net = wnet.WarmStartNet(wnet.WarmStartNet.init_params(0), dev)
-    data = wtrain.make_dataset(net, 64, seed=77, chunk=64, log=lambda s: None)
-    l0 = wtrain.evaluate(net, data, batch=16)
-    wtrain.train(net, data, epochs=15, batch=16, lr=1e-3, log=lambda s: print(s, flush=True))
-    l1 = wtrain.evaluate(net, data, batch=16)
+    data = wtrain.make_dataset(net, 8, seed=77, chunk=8, log=lambda s: None)
+    l0 = wtrain.evaluate(net, data, batch=8)
+    wtrain.train(net, data, epochs=150, batch=8, lr=2e-3, lr_min=2e-3, log=lambda s: None)
+    l1 = wtrain.evaluate(net, data, batch=8)
     print("loss before / after", l0, l1, flush=True)
-    assert l1 < 0.8 * l0, (l0, l1)
+    assert l1 < 0.6 * l0, (l0, l1)

@@ -61,7 +61,7 @@ def test_dense_argument_validation():
         return lib.potamd_sinkhorn_dense(
             args["dtype"], args["domain"], args["order"], args["kg"], args["M"],
             args["ldm"], args["bs"], args["B"], args["n"], args["m"], args["reg"],
-            P, P, P, P, args["max_iter"], 1e-9, args["err_norm"], P, 4, 0, 0, 0,
+            P, P, P, P, args["max_iter"], 1e-9, args["err_norm"], P, 4, 0, 0, 0, 0,
             args["ws"], args["wsb"], None, None, info)
 
     assert call() == -4                      # valid, but no workspace
