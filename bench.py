@@ -10,7 +10,12 @@ solver, all timed with the inputs already resident in HBM.
 
 Other workloads (``--workload``): ``dense`` (one 16384 x 16384 problem,
 d = 64, 200 fixed iterations, materialised 1 GiB cost), ``ragged`` (512
-uneven problems, n_i, m_i in [64, 1024]).
+uneven problems, n_i, m_i in [64, 1024]), ``online`` (the 16384 x 16384,
+d = 64 problem with the cost computed on the fly, 200 fixed iterations, one
+``--precision`` of the x . y^T term), ``precision`` (n = m = 65536, d = 128,
+reg = 5e-2, 100 iterations on the online cost in every precision against the
+full-size fp64 GPU run, and on a 4096-point subsample against the float64
+NumPy oracle), ``copy`` (1 GiB device copy, the HBM reference).
 
 Prints one JSON line; only N = 1 is supported.
 """
@@ -25,6 +30,7 @@ import numpy as np
 
 METRIC = "problems/sec to marginal err≤1e-6 (learned warm-start vs plain GPU Sinkhorn)"
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"f32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0, "fp8": 5000.0, "f64": 78.6}
 
 
 def _args():
@@ -32,7 +38,12 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="warmstart", choices=["warmstart", "dense", "ragged"])
+    ap.add_argument("--workload", default="warmstart", choices=["warmstart", "dense", "ragged", "copy", "online",
+                             "precision"])
+    ap.add_argument("--precision", default="bf16x3",
+                    choices=["f32", "bf16", "bf16x3", "fp8", "f64"])
+    ap.add_argument("--subsample", type=int, default=4096)
+    ap.add_argument("--graph", type=int, default=1, help="ragged: replay a captured hipGraph")
     ap.add_argument("--batch", type=int, default=4096, help="problems per step")
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per problem (0 = default)")
     ap.add_argument("--prefetch", type=int, default=0, help="rows prefetched per wave (0 = 1)")
@@ -202,33 +213,224 @@ def run_dense(args, torch, dev):
 
 
 def run_ragged(args, torch, dev):
+    """B = 512 uneven problems (n_i, m_i ~ U[64, 1024], d = 16, GMM(1-4)),
+    stopThr 1e-6, numItermax 2000: ragged distances + batched solver, replayed
+    as one captured hipGraph (``--graph``, default) or launched eagerly."""
     from pot_amd import problems, ragged
     fam = problems.config3(seed=2, B=512)
+    ns = [len(x) for x in fam.xs]
+    ms = [len(y) for y in fam.ys]
     X = [torch.tensor(x, device=dev) for x in fam.xs]
     Y = [torch.tensor(y, device=dev) for y in fam.ys]
+    Xp = torch.cat(X)
+    Yp = torch.cat(Y)
+    G = ragged.RaggedGraph(ns, ms, 16, numItermax=2000, stopThr=1e-6, check_every=10,
+                           waves=args.waves, prefetch=args.prefetch) if args.graph else None
     times, kms, abytes = [], [], []
+    res = None
     for k in range(args.warmup + args.steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        R = ragged.from_clouds(X, Y, fam.regs)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        res = ragged.sinkhorn_log_ragged(R, numItermax=2000, stopThr=1e-6)
-        e1.record()
+        if G is not None:
+            e0.record()
+            res = G.solve(Xp, Yp, fam.regs)
+            e1.record()
+        else:
+            R = ragged.from_clouds(X, Y, fam.regs)
+            e0.record()
+            res = ragged.sinkhorn_log_ragged(R, numItermax=2000, stopThr=1e-6,
+                                             waves=args.waves, prefetch=args.prefetch)
+            e1.record()
         torch.cuda.synchronize()
         if k >= args.warmup:
             times.append(time.perf_counter() - t0)
             kms.append(e0.elapsed_time(e1))
             nit = res.niter.cpu().numpy().astype(np.int64)
-            abytes.append(float(np.sum((nit + 2) * R.ns.astype(np.int64) * R.ms * 4)))
+            abytes.append(float(np.sum((nit + 2) * np.asarray(ns, np.int64) * np.asarray(ms))
+                                * 4))
     dt = float(np.mean(times))
     gbs = np.mean(abytes) / (np.mean(kms) * 1e-3) / 1e9
-    return {"value": 512 / dt, "unit": "problems/s", "ms_per_step": dt * 1e3, "dtype": "f32",
-            "config": {"workload": "ragged B=512 n_i,m_i~U[64,1024] d=16 reg~logU[5e-3,5e-2]"},
-            "roofline": {"bound": "hbm", "kernel": "batch_log_kernel", "achieved": gbs,
+    out = {"value": 512 / dt, "unit": "problems/s", "ms_per_step": dt * 1e3, "dtype": "f32",
+           "config": {"workload": "ragged B=512 n_i,m_i~U[64,1024] d=16 GMM(1-4) "
+                                  "reg~logU[5e-3,5e-2] stopThr=1e-6",
+                      "launch": "hipGraph replay" if G is not None else "eager",
+                      "seed": 2},
+           "iters": {"mean": float(nit.mean()), "max": int(nit.max())},
+           "roofline": {"bound": "hbm", "kernel": "batch_log_kernel (+ragged dist in the "
+                        "graph)", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "traffic": None}}
+    if not args.no_cpu:
+        # bounded sample: problems 0-7 by the float64 oracle; the same run gives
+        # the per-problem plan error of the GPU solution
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        import oracle.sinkhorn as osk
+        u_all, v_all = res.u.cpu().numpy(), res.v.cpu().numpy()
+        ao = np.concatenate([[0], np.cumsum(ns)[:-1]])
+        bo = np.concatenate([[0], np.cumsum(ms)[:-1]])
+        tot, worst = 0.0, 0.0
+        for q in range(8):
+            x64 = fam.xs[q].astype(np.float64)
+            y64 = fam.ys[q].astype(np.float64)
+            M = np.maximum((x64 ** 2).sum(1)[:, None] + (y64 ** 2).sum(1)[None]
+                           - 2 * x64 @ y64.T, 0.0)
+            M /= M.max()
+            a = np.full(ns[q], 1.0 / ns[q])
+            b = np.full(ms[q], 1.0 / ms[q])
+            t0 = time.perf_counter()
+            Pr = osk.sinkhorn_log(a, b, M, fam.regs[q], numItermax=2000, stopThr=1e-6,
+                                  warn=False)
+            tot += time.perf_counter() - t0
+            u = u_all[ao[q]:ao[q] + ns[q]]
+            v = v_all[bo[q]:bo[q] + ms[q]]
+            P = np.exp(-M / fam.regs[q] + u[:, None] + v[None, :])
+            worst = max(worst, float(np.max(np.abs(P - Pr)) / Pr.max()))
+        out["cpu_baseline"] = {"value": 8 / tot, "unit": "problems/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "problems 0-7, float64 NumPy sinkhorn_log"}
+        out["plan_err_vs_cpu"] = {"max_abs_over_max_entry": worst, "problems": 8}
+    return out
+
+
+def run_online(args, torch, dev):
+    """16384 x 16384, d = 64, reg = 1e-2: the dense problem with the cost on the fly."""
+    from pot_amd import online, problems
+    x, y, a, b = problems.config2_clouds()
+    n, d = x.shape
+    X = torch.tensor(x, device=dev)
+    Y = torch.tensor(y, device=dev)
+    at = torch.tensor(a, device=dev)
+    bt = torch.tensor(b, device=dev)
+    its = 200
+    # the normalisation (max cost) once, outside the timed region
+    _, _, lg = online.sinkhorn_online(X, Y, 1e-2, at, bt, numItermax=1, stopThr=0.0,
+                                      precision=args.precision, warn=False)
+    cs = lg["cost_scale"]
+
+    def solve():
+        return online.sinkhorn_online(X, Y, 1e-2, at, bt, numItermax=its, stopThr=0.0,
+                                      precision=args.precision, cost_scale=cs, warn=False)
+
+    for _ in range(args.warmup):
+        solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        u, v, lg = solve()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    passes = 2 * its + 1
+    flops = 2.0 * n * n * d * passes * (3 if args.precision == "bf16x3" else 1)
+    tf = flops / dt / 1e12
+    peak = MFMA_PEAK_TFLOPS[args.precision]
+    _, sums = online.online_plan(X, Y, 1e-2, u, v, cs, at, bt)
+    return {"value": its / dt, "unit": "iterations/s", "ms_per_step": dt * 1e3,
+            "dtype": args.precision,
+            "config": {"workload": "online n=m=16384 d=64 reg=1e-2 200 fixed iterations",
+                       "precision": args.precision, "cost_scale": cs},
+            "exp_per_s": float(n) * n * passes / dt,
+            "check_fp64": sums,
+            "roofline": {"bound": "mfma", "kernel": "online_pass_kernel (+merge)",
+                         "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak,
+                         "traffic": None,
+                         "note": "whole-iteration time; 2 n m d flops per pass (x3 for the "
+                                 "split), 2 passes per iteration + 1 prologue; the pass is "
+                                 "bound by exp2/VALU work (n m exp2 per pass), not MFMA"}}
+
+
+def run_precision(args, torch, dev):
+    """Precision study: n = m = 65536, d = 128, reg = 5e-2, 100 iterations."""
+    from pot_amd import online, problems
+    x, y = problems.config4_clouds()
+    n, d = x.shape
+    X = torch.tensor(x, device=dev)
+    Y = torch.tensor(y, device=dev)
+    reg, its = 5e-2, 100
+    t_all = time.perf_counter()
+    rows = {}
+    ref = None
+    for prec in ["f64", "f32", "bf16x3", "bf16", "fp8"]:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u, v, lg = online.sinkhorn_online(X, Y, reg, numItermax=its, stopThr=0.0,
+                                          precision=prec, warn=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        _, sums = online.online_plan(X, Y, reg, u, v, lg["cost_scale"])
+        row = {"s": dt, "iterations_per_s": its / dt, "cost": sums["cost"],
+               "err_a": sums["err_a"], "err_b": sums["err_b"], "cost_scale": lg["cost_scale"]}
+        if ref is None:
+            ref = (u, v, sums["cost"])
+        else:
+            du = (u - ref[0]).cpu().numpy()
+            dv = (v - ref[1]).cpu().numpy()
+            g = du.mean()
+            row["max_logplan_err"] = float(np.max(np.abs(du - g)) + np.max(np.abs(dv + g)))
+            row["cost_rel_err"] = abs(sums["cost"] - ref[2]) / ref[2]
+        rows[prec] = row
+        print(f"[precision] {prec}: {json.dumps(row)}", file=sys.stderr, flush=True)
+    # subsample against the float64 NumPy oracle
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle.sinkhorn as osk
+    k = args.subsample
+    xs, ys = x[:k], y[:k]
+    M = ((xs.astype(np.float64) ** 2).sum(1)[:, None] + (ys.astype(np.float64) ** 2).sum(1)[None]
+         - 2.0 * xs.astype(np.float64) @ ys.astype(np.float64).T)
+    M = np.maximum(M, 0.0)
+    M /= M.max()
+    ab = np.full(k, 1.0 / k)
+    t0 = time.perf_counter()
+    Pr = osk.sinkhorn_log(ab, ab, M, reg, numItermax=its, stopThr=0.0, warn=False)
+    t_cpu = time.perf_counter() - t0
+    cost_ref = float(np.sum(Pr * M))
+    sub = {}
+    for prec in ["f64", "f32", "bf16x3", "bf16", "fp8"]:
+        u, v, lg = online.sinkhorn_online(xs, ys, reg, numItermax=its, stopThr=0.0,
+                                          precision=prec, warn=False)
+        P, sums = online.online_plan(xs, ys, reg, u, v, lg["cost_scale"], want_plan=True)
+        Pn = P.cpu().numpy()
+        mask = Pr >= 1e-30
+        sub[prec] = {"plan_rel_err": float(np.max(np.abs(Pn[mask] - Pr[mask]) / Pr[mask])),
+                     "cost_rel_err": abs(sums["cost"] - cost_ref) / cost_ref,
+                     "err_b": sums["err_b"]}
+        print(f"[precision] subsample {prec}: {json.dumps(sub[prec])}", file=sys.stderr,
+              flush=True)
+    best = rows["bf16x3"]
+    return {"value": best["iterations_per_s"], "unit": "iterations/s",
+            "ms_per_step": 1e3 * best["s"] / its, "dtype": "bf16x3",
+            "config": {"workload": "precision n=m=65536 d=128 reg=5e-2 100 iterations, online cost",
+                       "subsample": k},
+            "full_size_vs_f64_gpu": rows, "subsample_vs_numpy_oracle": sub,
+            "cpu_baseline": {"value": its / t_cpu, "unit": "iterations/s", "cores": 1,
+                             "kind": "port",
+                             "sample": f"float64 NumPy sinkhorn_log on the {k}-point subsample"},
+            "wall_s": time.perf_counter() - t_all}
+
+
+def run_copy(args, torch, dev):
+    """Device-to-device copy of 1 GiB: the HBM reference of the stream kernels and
+    the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    nbytes = 1 << 30
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    return {"value": gbs, "unit": "GB/s", "ms_per_step": ms, "dtype": "f32",
+            "config": {"workload": "device copy 1 GiB (read 1 GiB + write 1 GiB)"},
+            "roofline": {"bound": "hbm", "kernel": "copy", "achieved": gbs,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                         "traffic": None}}
+                         "traffic": 2 * nbytes}}
 
 
 def main():
@@ -242,7 +444,9 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
     dev = torch.device("cuda", 0)
-    runner = {"warmstart": run_warmstart, "dense": run_dense, "ragged": run_ragged}[args.workload]
+    runner = {"warmstart": run_warmstart, "dense": run_dense, "ragged": run_ragged,
+              "copy": run_copy, "online": run_online,
+              "precision": run_precision}[args.workload]
     res = runner(args, torch, dev)
     line = {"metric": METRIC, "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),

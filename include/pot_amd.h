@@ -199,6 +199,51 @@ int potamd_mlp_op(int op, int rows, int seg_len, int P, int k, int n, int64_t co
                   double scalar, const float* hp, const void* x, const void* y, const void* z,
                   const void* w, void* out, void* out2, void* stream);
 
+/* ---- Online cost: squared Euclidean cost of two point clouds, never stored ----
+ *
+ * Replaces the loop of sinkhorn_log (ot/bregman/_sinkhorn.py:880-906) for
+ * M = dist(X, Y) / scale (ot/utils.py:400-466, euclidean_distances :270-307)
+ * without materialising M; the reference's own lazy path is
+ * empirical_sinkhorn(isLazy=True) (ot/bregman/_empirical.py:193-270).
+ * Precision of the x . y^T term: */
+#define POTAMD_ONLINE_F32 0    /* fp32 MFMA */
+#define POTAMD_ONLINE_BF16 1   /* bf16 MFMA, fp32 accumulation */
+#define POTAMD_ONLINE_BF16X3 2 /* bf16 hi + lo split, three MFMA products */
+#define POTAMD_ONLINE_FP8 3    /* fp8 e4m3 MFMA, per-point scale */
+#define POTAMD_ONLINE_F64 4    /* fp64 vector ALU (reference precision) */
+
+/* Workspace of potamd_sinkhorn_online; 0 for unsupported arguments (the MFMA
+ * precisions take d <= 128). */
+size_t potamd_online_workspace_size(int precision, int n, int m, int d);
+
+/* sinkhorn_log on the online cost, column update first, marginal error
+ * ||P^T 1 - b||_2 every 10 iterations, strict `err < stop_thr`, exactly as
+ * _sinkhorn.py:880-899.  X (n x d, row stride ldx) and Y (m x d) are fp32
+ * device points; a, b, u (in: u0, out: log_u), v (out: log_v) fp64 device,
+ * natural logs.  cost_scale > 0 divides the cost by it; 0 divides by the
+ * maximum cost, computed on the device and returned in *cost_scale_out
+ * (host pointer, may be NULL).  splits: C-side splits per pass (0 = auto,
+ * at most 64).  info as potamd_sinkhorn_dense.  Returns the status. */
+int potamd_sinkhorn_online(int precision, const float* X, int64_t ldx, const float* Y, int64_t ldy,
+                           int n, int m, int d, double reg, double cost_scale, const double* a,
+                           const double* b, double* u, double* v, int max_iter, double stop_thr,
+                           double* err_hist, int err_cap, int splits, void* workspace,
+                           size_t ws_bytes, void* host_ctrl, void* stream, int* info,
+                           double* cost_scale_out);
+
+/* Workspace of potamd_online_plan. */
+size_t potamd_online_plan_workspace_size(int n, int m);
+
+/* Plan, cost and marginals of the online cost in fp64 (_sinkhorn.py:915-918
+ * and the cost of sinkhorn2, :436): P_ij = exp(-M_ij / reg + u_i + v_j) with
+ * M_ij from the fp32 points in fp64 (clamped at 0, as ot/utils.py:299).
+ * P (n x ldp fp64) may be NULL.  sums (device, 4 doubles) = [<P, M>, sum P,
+ * ||P 1 - a||_2, ||P^T 1 - b||_2]. */
+int potamd_online_plan(const float* X, int64_t ldx, const float* Y, int64_t ldy, int n, int m,
+                       int d, double reg, double cost_scale, const double* u, const double* v,
+                       const double* a, const double* b, double* P, int64_t ldp, double* sums,
+                       void* workspace, size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

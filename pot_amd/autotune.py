@@ -29,7 +29,7 @@ ROW_CONFIGS = {
 }
 RPB_CHOICES = (8, 16, 32, 64, 128)
 
-_lock = threading.Lock()
+_lock = threading.RLock()
 _mem = None
 
 

@@ -32,13 +32,20 @@ CFLAGS = [
 ]
 
 
+# Per-file flags.  The online-cost pass keeps its MFMA accumulators in VGPRs,
+# so its exp2 epilogue reads them without accvgpr copies.
+EXTRA = {
+    "online.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def _sources():
     return sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
 def _compile(src):
     obj = os.path.join(OBJ, src[:-4] + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *CFLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     t0 = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

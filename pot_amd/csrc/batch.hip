@@ -165,7 +165,7 @@ restart:
           cur[q] = exp2_fast(scaled_exponent(cur[q], s_hi, s_lo, t));
           s += cur[q];
         }
-        float S = wave_sum(s);
+        float S = wave_sum_dpp(s);
         if (!in_safe_range((double)S)) {
           load_row(i, cur);
           float mxr = -__builtin_inff();
@@ -184,7 +184,7 @@ restart:
             cur[q] = exp2_fast(scaled_exponent(cur[q], s_hi, s_lo, t));
             s += cur[q];
           }
-          S = wave_sum(s);
+          S = wave_sum_dpp(s);
           if (lane == 0) c2s[i] = c2;  // the shift actually used
         }
         if (lane == 0) sh[i] = S;

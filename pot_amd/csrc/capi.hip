@@ -13,30 +13,12 @@
 #include <algorithm>
 
 #include "../../include/pot_amd.h"
+#include "host.h"
 #include "internal.h"
 
 using namespace potamd;
 
-namespace {
-
-constexpr size_t kAlign = 256;
-constexpr int kRing = 4;
-constexpr int kColLseRows = 256;
-
-size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
-
-struct Carver {
-  char* base;
-  size_t off = 0;
-  explicit Carver(void* b) : base(static_cast<char*>(b)) {}
-  template <typename T> T* take(size_t count) {
-    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
-    off += align_up(count * sizeof(T));
-    return p;
-  }
-};
-
-int device_cus() {
+int potamd::device_cus() {
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;
@@ -47,6 +29,11 @@ int device_cus() {
   }
   return cus;
 }
+
+namespace {
+
+constexpr int kRing = 4;
+constexpr int kColLseRows = 256;
 
 int default_rpb(int n) {
   // About two workgroups per CU, at least 4 rows each (the column partials
@@ -85,7 +72,7 @@ DenseLayout carve_dense(void* ws, int dtype, int B, int n, int m, int rpb) {
   L.logb2 = c.take<double>((size_t)B * m);
   L.colpart = c.take<char>((size_t)B * G * m * es);
   L.rowerr = c.take<double>((size_t)B * G);
-  L.errpart = c.take<double>((size_t)B * ((m + 255) / 256));
+  L.errpart = c.take<double>((size_t)B * ((m + 63) / 64));  // one per merge block
   L.pmax = c.take<double>((size_t)B * G2 * m);
   L.psum = c.take<double>((size_t)B * G2 * m);
   L.ctrl = c.take<Ctrl>(1);
@@ -93,15 +80,6 @@ DenseLayout carve_dense(void* ws, int dtype, int B, int n, int m, int rpb) {
   L.bytes = c.off;
   return L;
 }
-
-#define HIP_TRY(expr)                       \
-  do {                                      \
-    hipError_t _e = (expr);                 \
-    if (_e != hipSuccess) return POTAMD_EHIP; \
-  } while (0)
-
-constexpr double kLog2e = 1.4426950408889634074;
-constexpr double kLn2 = 0.69314718055994530942;
 
 void split_f32(double x, float* hi, float* lo) {
   *hi = (float)x;

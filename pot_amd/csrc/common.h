@@ -57,6 +57,31 @@ __device__ __forceinline__ double wave_sum(double x) {
   for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
   return x;
 }
+// Wave sum through DPP (no LDS round trips): four row-local steps give every
+// lane its 16-lane row sum, then the four row sums are read from lanes 0, 16,
+// 32, 48 and added in a fixed order.  The result is wave-uniform and
+// deterministic.  DPP controls (GFX9): quad_perm [1,0,3,2] = 0xB1,
+// quad_perm [2,3,0,1] = 0x4E, row_half_mirror = 0x141, row_mirror = 0x140.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float wave_sum_dpp(float x) {
+  x += dpp_f<0xB1>(x);
+  x += dpp_f<0x4E>(x);
+  x += dpp_f<0x141>(x);
+  x += dpp_f<0x140>(x);
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 0));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 16));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 48));
+  return (r0 + r1) + (r2 + r3);
+}
+
+__device__ __forceinline__ float wave_sum_fast(float x) { return wave_sum_dpp(x); }
+__device__ __forceinline__ double wave_sum_fast(double x) { return wave_sum(x); }
+
 __device__ __forceinline__ float wave_max(float x) {
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, kWave));

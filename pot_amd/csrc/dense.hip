@@ -26,7 +26,7 @@ namespace potamd {
 
 template <int W, typename R>
 __device__ __forceinline__ R block_sum(R x, R* red, int& parity) {
-  x = wave_sum(x);
+  x = wave_sum_fast(x);
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   if (lane == 0) red[parity * W + wid] = x;
   lds_barrier();
@@ -290,25 +290,46 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
 // Column merge + marginal check + column update.  The last workgroup to
 // finish (ticket counter) reduces the per-block error partials in a fixed
 // order and takes the stop decision, so results are deterministic.
+// Column merge: a block owns kMergeCols columns; its 256 threads split the G
+// row-block partials of each column into 256 / kMergeCols groups, summed in
+// LDS (fixed order, so the result is deterministic).
+constexpr int kMergeCols = 64;
+
 template <typename TP>
 __global__ __launch_bounds__(256) void col_merge_kernel(MergeArgs A) {
   Ctrl* ctrl = A.ctrl;
   if (ctrl->done) return;
+  constexpr int NG = 256 / kMergeCols;
+  __shared__ double gsum[NG][kMergeCols];
   const int p = blockIdx.y;
   const int tid = threadIdx.x;
-  const int j = blockIdx.x * 256 + tid;
+  const int cl = tid % kMergeCols, grp = tid / kMergeCols;
+  const int j = blockIdx.x * kMergeCols + cl;
   const int m = A.m;
   const bool col_first = A.order == kColFirst;
   const bool check_col = !A.prologue && col_first && (A.iter % 10 == 0);
   const bool check_row = !A.prologue && !col_first && A.iter >= 1 && ((A.iter - 1) % 10 == 0);
   const int next_slot = (A.iter + 1) & 1;
 
+  {
+    double part = 0.0;
+    if (j < m) {
+      const TP* cp = static_cast<const TP*>(A.colpart) + (int64_t)p * A.G * m + j;
+      const int g0 = grp * ((A.G + NG - 1) / NG);
+      const int g1 = min(A.G, g0 + (A.G + NG - 1) / NG);
+#pragma unroll 8
+      for (int g = g0; g < g1; ++g) part += (double)cp[(int64_t)g * m];
+    }
+    gsum[grp][cl] = part;
+  }
+  __syncthreads();
+
   double e2 = 0.0;
   int bad = 0, need_exact = 0;
-  if (j < m) {
-    const TP* cp = static_cast<const TP*>(A.colpart) + (int64_t)p * A.G * m + j;
+  if (grp == 0 && j < m) {
     double cs = 0.0;
-    for (int g = 0; g < A.G; ++g) cs += (double)cp[(int64_t)g * m];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) cs += gsum[g][cl];
     const double bj = A.b[(int64_t)p * m + j];
     const double vc = A.v_cur[(int64_t)p * m + j];
     double vn;
@@ -485,8 +506,10 @@ __global__ void ctrl_init_kernel(Ctrl* c, int* bad) {
     c->need_exact = 0;
     c->pad1 = 0;
     c->err = 0.0;
-    bad[0] = 0;
-    bad[1] = 0;
+    if (bad) {
+      bad[0] = 0;
+      bad[1] = 0;
+    }
   }
 }
 
@@ -593,7 +616,7 @@ hipError_t launch_row_sweep(int dtype, int vec_ok, int T, int cpt, int pf, int d
 }
 
 hipError_t launch_col_merge(int dtype, const MergeArgs& a, int B, hipStream_t s) {
-  dim3 grid((a.m + 255) / 256, B);
+  dim3 grid((a.m + kMergeCols - 1) / kMergeCols, B);
   if (dtype == 0)
     hipLaunchKernelGGL(col_merge_kernel<float>, grid, dim3(256), 0, s, a);
   else

@@ -80,6 +80,15 @@ def config2_clouds(seed=1, n=16384, d=64):
     return x.astype(np.float32), y.astype(np.float32), a, b
 
 
+def config4_clouds(seed=3, n=65536, d=128):
+    """x ~ N(0, I), y ~ N(0, I) + 0.25 * U[-1, 1]^d (one shift vector), fp32."""
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, d))
+    shift = 0.25 * rng.uniform(-1.0, 1.0, size=d)
+    y = rng.standard_normal((n, d)) + shift
+    return x.astype(np.float32), y.astype(np.float32)
+
+
 def config3(seed=2, B=512):
     return ragged_family(B, seed, n_range=(64, 1024), d=16, kmin=1, kmax=4)
 

@@ -213,3 +213,96 @@ def sinkhorn_log_ragged(R, u0=None, numItermax=1000, stopThr=1e-9, order=None,
     _lib.check(rc, "potamd_sinkhorn_log_batch")
     return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
                         err_last=e_last)
+
+
+class RaggedGraph:
+    """Cost construction and solve of a fixed ragged layout, captured once as a
+    hipGraph (through ``torch.cuda.CUDAGraph``) and replayed per batch.
+
+    The layout (sizes, d, iteration settings) is fixed at construction; each
+    ``solve`` copies new points, regs and (optionally) initial potentials into
+    the graph's static buffers, sets the launch order, and replays the two
+    kernels (ragged squared distances with per-problem maxima, then the
+    batched sinkhorn_log).  The captured work contains no memset nodes and no
+    host synchronisation.  Results are views of static buffers, overwritten
+    by the next ``solve``.
+    """
+
+    def __init__(self, ns, ms, d, numItermax=1000, stopThr=1e-9, check_every=10,
+                 fallback_thr=0.0, waves=0, prefetch=0, device=None):
+        self.device = device or _device.require_gpu()
+        dev = self.device
+        self.lib = _lib.load()
+        (self.ns, self.ms, total, self.moff, self.ns_d, self.ms_d, self.ld_d, self.aoff,
+         self.boff) = _layout(ns, ms, dev)
+        B = len(self.ns)
+        self.d = int(d)
+        tn, tm = int(self.ns.sum()), int(self.ms.sum())
+        self.X = torch.zeros((tn, self.d), dtype=torch.float32, device=dev)
+        self.Y = torch.zeros((tm, self.d), dtype=torch.float32, device=dev)
+        self.M = torch.empty(total, dtype=torch.float32, device=dev)
+        self.cmax = torch.empty(B, dtype=torch.float32, device=dev)
+        self.reg_in = torch.ones(B, dtype=torch.float64, device=dev)
+        self.reg_eff = torch.ones(B, dtype=torch.float64, device=dev)
+        self.a = _weights(None, self.ns, dev)
+        self.b = _weights(None, self.ms, dev)
+        self.u0 = torch.zeros(tn, dtype=torch.float64, device=dev)
+        self.u = torch.zeros(tn, dtype=torch.float64, device=dev)
+        self.v = torch.zeros(tm, dtype=torch.float64, device=dev)
+        self.niter = torch.zeros(B, dtype=torch.int32, device=dev)
+        self.status = torch.zeros(B, dtype=torch.int32, device=dev)
+        self.e_first = torch.zeros(B, dtype=torch.float64, device=dev)
+        self.e_last = torch.zeros(B, dtype=torch.float64, device=dev)
+        self.order = torch.arange(B, dtype=torch.int32, device=dev)
+        self.settings = (int(numItermax), int(check_every), float(stopThr), float(fallback_thr),
+                         int(waves) | (int(prefetch) << 8))
+        # eager warm-up on a side stream, then capture
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            self._enqueue()
+        torch.cuda.current_stream(dev).wait_stream(side)
+        torch.cuda.synchronize(dev)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._enqueue()
+        torch.cuda.synchronize(dev)
+
+    def _enqueue(self):
+        dev = self.device
+        stream = ctypes.c_void_p(_device.stream_ptr(dev))
+        rc = self.lib.potamd_sqeuclidean_ragged(
+            len(self.ns), int(self.ns.max()), int(self.ms.max()), self.d, _ptr(self.X),
+            _ptr(self.Y), _ptr(self.aoff), _ptr(self.boff), _ptr(self.ns_d), _ptr(self.ms_d),
+            _ptr(self.M), _ptr(self.moff), _ptr(self.ld_d), _ptr(self.cmax), stream)
+        _lib.check(rc, "potamd_sqeuclidean_ragged")
+        torch.mul(self.reg_in, self.cmax.to(torch.float64), out=self.reg_eff)
+        self.u.copy_(self.u0)
+        max_iter, check_every, stop, fb, wv = self.settings
+        rc = self.lib.potamd_sinkhorn_log_batch(
+            _lib.F32, len(self.ns), int(self.ns.max()), int(self.ms.max()), _ptr(self.M),
+            _ptr(self.moff), _ptr(self.ns_d), _ptr(self.ms_d), _ptr(self.ld_d), _ptr(self.aoff),
+            _ptr(self.boff), _ptr(self.reg_eff), _ptr(self.a), _ptr(self.b), _ptr(self.u),
+            _ptr(self.v), max_iter, check_every, stop, fb, _ptr(self.order), 1,
+            _ptr(self.niter), _ptr(self.status), _ptr(self.e_last), _ptr(self.e_first), wv,
+            stream)
+        _lib.check(rc, "potamd_sinkhorn_log_batch")
+
+    def solve(self, xs, ys, regs, u0=None, order=None):
+        """Replay on new data: xs, ys packed (sum n, d) / (sum m, d) or (B, n, d)
+        tensors; regs (B,) host values.  Returns a RaggedResult of views."""
+        self.X.copy_(_pack_points(xs, self.device).view_as(self.X))
+        self.Y.copy_(_pack_points(ys, self.device).view_as(self.Y))
+        regs = np.asarray(regs, dtype=np.float64)
+        self.reg_in.copy_(torch.from_numpy(regs))
+        if u0 is None:
+            self.u0.zero_()
+        else:
+            self.u0.copy_(u0)
+        if order is None:
+            work = self.ns.astype(np.float64) * self.ms / regs
+            order = np.argsort(-work, kind="stable").astype(np.int32)
+        self.order.copy_(torch.as_tensor(np.asarray(order, dtype=np.int32)))
+        self.graph.replay()
+        return RaggedResult(u=self.u, v=self.v, niter=self.niter, status=self.status,
+                            err_first=self.e_first, err_last=self.e_last)
