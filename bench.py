@@ -47,6 +47,8 @@ def _args():
     ap.add_argument("--batch", type=int, default=4096, help="problems per step")
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per problem (0 = default)")
     ap.add_argument("--prefetch", type=int, default=0, help="rows prefetched per wave (0 = 1)")
+    ap.add_argument("--occ", type=int, default=0,
+                    help="batch kernel register cap in waves/SIMD (0 default, 4, 255 none)")
     ap.add_argument("--check-every", type=int, default=1,
                     help="iterations between marginal checks (1: stop at the first iterate "
                          "with err <= stopThr; 10: the reference's cadence)")
@@ -99,6 +101,7 @@ def run_warmstart(args, torch, dev):
                                              stopThr=args.stop, waves=args.waves,
                                              prefetch=args.prefetch,
                                              check_every=args.check_every,
+                                             occupancy=args.occ,
                                              fallback_thr=ws_mod.FALLBACK_THR if warm else 0.0)
             e1.record()
             torch.cuda.synchronize()
@@ -225,7 +228,8 @@ def run_ragged(args, torch, dev):
     Xp = torch.cat(X)
     Yp = torch.cat(Y)
     G = ragged.RaggedGraph(ns, ms, 16, numItermax=2000, stopThr=1e-6, check_every=10,
-                           waves=args.waves, prefetch=args.prefetch) if args.graph else None
+                           waves=args.waves, prefetch=args.prefetch,
+                           occupancy=args.occ) if args.graph else None
     times, kms, abytes = [], [], []
     res = None
     for k in range(args.warmup + args.steps):
@@ -241,7 +245,8 @@ def run_ragged(args, torch, dev):
             R = ragged.from_clouds(X, Y, fam.regs)
             e0.record()
             res = ragged.sinkhorn_log_ragged(R, numItermax=2000, stopThr=1e-6,
-                                             waves=args.waves, prefetch=args.prefetch)
+                                             waves=args.waves, prefetch=args.prefetch,
+                                             occupancy=args.occ)
             e1.record()
         torch.cuda.synchronize()
         if k >= args.warmup:

@@ -157,7 +157,9 @@ int potamd_sqeuclidean_ragged(int batch, int max_n, int max_m, int d, const floa
  * niter, status: per-problem outputs (device int arrays); err_last / err_first:
  *    last and first checked marginal errors (device, may be NULL).
  * waves: wavefronts per problem (4 or 8; 0 = default) in the low byte, rows
- *    prefetched per wave (1 or 2, 0 = 1; 2 needs max_m > 512) in the next byte.
+ *    prefetched per wave (1 or 2, 0 = 1; 2 needs max_m > 512) in the next byte,
+ *    register cap in waves per SIMD in the third (0 = default, 4 = two 8-wave
+ *    workgroups per CU (8 waves, 1 row prefetched only), 255 = no cap).
  */
 int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int max_m, const void* M,
                               const int64_t* moff, const int* ns, const int* ms, const int* ldms,
@@ -218,16 +220,23 @@ size_t potamd_online_workspace_size(int precision, int n, int m, int d);
 
 /* sinkhorn_log on the online cost, column update first, marginal error
  * ||P^T 1 - b||_2 every 10 iterations, strict `err < stop_thr`, exactly as
- * _sinkhorn.py:880-899.  X (n x d, row stride ldx) and Y (m x d) are fp32
- * device points; a, b, u (in: u0, out: log_u), v (out: log_v) fp64 device,
- * natural logs.  cost_scale > 0 divides the cost by it; 0 divides by the
+ * _sinkhorn.py:880-899.  X (n x d, row stride ldx) and Y (m x d) are device
+ * points of `dtype` (POTAMD_F64 only with POTAMD_ONLINE_F64); a, b, u (in:
+ * u0, out: log_u), v (out: log_v) fp64 device, natural logs.  cost_scale > 0 divides the cost by it; 0 divides by the
  * maximum cost, computed on the device and returned in *cost_scale_out
  * (host pointer, may be NULL).  splits: C-side splits per pass (0 = auto,
- * at most 64).  info as potamd_sinkhorn_dense.  Returns the status. */
-int potamd_sinkhorn_online(int precision, const float* X, int64_t ldx, const float* Y, int64_t ldy,
-                           int n, int m, int d, double reg, double cost_scale, const double* a,
+ * at most 64).  semantics 0 is the loop above; 1 is the loop of
+ * empirical_sinkhorn(isLazy=True) (ot/bregman/_empirical.py:218-256) seen
+ * from its second point set (call it with X = X_t, Y = X_s, a = b_t, b = a_s,
+ * u0 = g0): the error is the 1-norm, checked when (ii + 1) % 10 == 0, with a
+ * non-strict `err <= stop_thr`.  info as potamd_sinkhorn_dense.  Returns the
+ * status. */
+int potamd_sinkhorn_online(int precision, int dtype, const void* X, int64_t ldx, const void* Y,
+                           int64_t ldy, int n, int m, int d, double reg, double cost_scale,
+                           const double* a,
                            const double* b, double* u, double* v, int max_iter, double stop_thr,
-                           double* err_hist, int err_cap, int splits, void* workspace,
+                           int semantics, double* err_hist, int err_cap, int splits,
+                           void* workspace,
                            size_t ws_bytes, void* host_ctrl, void* stream, int* info,
                            double* cost_scale_out);
 
@@ -236,11 +245,11 @@ size_t potamd_online_plan_workspace_size(int n, int m);
 
 /* Plan, cost and marginals of the online cost in fp64 (_sinkhorn.py:915-918
  * and the cost of sinkhorn2, :436): P_ij = exp(-M_ij / reg + u_i + v_j) with
- * M_ij from the fp32 points in fp64 (clamped at 0, as ot/utils.py:299).
+ * M_ij from the points (fp32 or fp64, `dtype`) in fp64 (clamped at 0, as ot/utils.py:299).
  * P (n x ldp fp64) may be NULL.  sums (device, 4 doubles) = [<P, M>, sum P,
  * ||P 1 - a||_2, ||P^T 1 - b||_2]. */
-int potamd_online_plan(const float* X, int64_t ldx, const float* Y, int64_t ldy, int n, int m,
-                       int d, double reg, double cost_scale, const double* u, const double* v,
+int potamd_online_plan(int dtype, const void* X, int64_t ldx, const void* Y, int64_t ldy, int n,
+                       int m, int d, double reg, double cost_scale, const double* u, const double* v,
                        const double* a, const double* b, double* P, int64_t ldp, double* sums,
                        void* workspace, size_t ws_bytes, void* stream);
 

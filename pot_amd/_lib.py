@@ -53,15 +53,16 @@ SIGNATURES = {
         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "potamd_online_workspace_size": (c_size, [c_int, c_int, c_int, c_int]),
     "potamd_sinkhorn_online": (c_int, [
-        c_int, c_vp, c_i64, c_vp, c_i64,     # precision, X, ldx, Y, ldy
+        c_int, c_int, c_vp, c_i64, c_vp, c_i64,  # precision, dtype, X, ldx, Y, ldy
         c_int, c_int, c_int, c_dbl, c_dbl,   # n, m, d, reg, cost_scale
         c_vp, c_vp, c_vp, c_vp,              # a, b, u, v
-        c_int, c_dbl, c_vp, c_int, c_int,    # max_iter, stop_thr, err_hist, err_cap, splits
+        c_int, c_dbl, c_int,                 # max_iter, stop_thr, semantics
+        c_vp, c_int, c_int,                  # err_hist, err_cap, splits
         c_vp, c_size, c_vp, c_vp, c_ip,      # workspace, ws_bytes, host_ctrl, stream, info
         ctypes.POINTER(c_dbl)]),             # cost_scale_out
     "potamd_online_plan_workspace_size": (c_size, [c_int, c_int]),
     "potamd_online_plan": (c_int, [
-        c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int, c_dbl, c_dbl,
+        c_int, c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int, c_dbl, c_dbl,
         c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_size, c_vp]),
     "potamd_sqeuclidean": (c_int, [
         c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_i64, c_i64, c_vp, c_i64,

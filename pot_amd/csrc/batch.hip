@@ -27,8 +27,13 @@ namespace potamd {
 
 constexpr int kBatchMaxN = 1024;
 
-template <int CPT, int VEC, int W, int PF>
-__global__ __launch_bounds__(W * kWave) void batch_log_kernel(BatchArgs A) {
+// OCC > 0 caps the registers so that OCC waves fit per SIMD (two 8-wave
+// workgroups per CU at OCC = 4); the few values this pushes to scratch are
+// per-problem and per-iteration scalars, none is touched in the row loop.
+template <int CPT, int VEC, int W, int PF, int OCC>
+__global__ __launch_bounds__(W * kWave)
+__attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, OCC > 0 ? OCC : 8)))
+void batch_log_kernel(BatchArgs A) {
   constexpr int T = W * kWave;
   constexpr int NCH = CPT / VEC;
   constexpr int MMAX = kWave * CPT;
@@ -287,25 +292,30 @@ int batch_cpt(int max_m) {
 
 int batch_waves_default() { return 8; }
 
-hipError_t launch_batch_log(int cpt, int vec_ok, int waves, int pf, const BatchArgs& a, int B,
-                            hipStream_t s) {
+int batch_occupancy_default(int waves, int pf) { return (waves == 8 && pf == 1) ? 4 : 0; }
+
+hipError_t launch_batch_log(int cpt, int vec_ok, int waves, int pf, int occ,
+                            const BatchArgs& a, int B, hipStream_t s) {
   dim3 grid(B);
-#define POTAMD_BATCH_CASE(C, WV, PP)                                                          \
-  if (cpt == C && waves == WV && pf == PP) {                                                 \
-    if (vec_ok)                                                                              \
-      hipLaunchKernelGGL((batch_log_kernel<C, 4, WV, PP>), grid, dim3(WV * kWave), 0, s, a); \
-    else                                                                                     \
-      hipLaunchKernelGGL((batch_log_kernel<C, 1, WV, PP>), grid, dim3(WV * kWave), 0, s, a); \
-    return hipGetLastError();                                                                \
+#define POTAMD_BATCH_CASE(C, WV, PP, OC)                                                          \
+  if (cpt == C && waves == WV && pf == PP && occ == OC) {                                        \
+    if (vec_ok)                                                                                  \
+      hipLaunchKernelGGL((batch_log_kernel<C, 4, WV, PP, OC>), grid, dim3(WV * kWave), 0, s, a); \
+    else                                                                                         \
+      hipLaunchKernelGGL((batch_log_kernel<C, 1, WV, PP, OC>), grid, dim3(WV * kWave), 0, s, a); \
+    return hipGetLastError();                                                                    \
   }
-  POTAMD_BATCH_CASE(4, 4, 1)
-  POTAMD_BATCH_CASE(8, 4, 1)
-  POTAMD_BATCH_CASE(16, 4, 1)
-  POTAMD_BATCH_CASE(4, 8, 1)
-  POTAMD_BATCH_CASE(8, 8, 1)
-  POTAMD_BATCH_CASE(16, 8, 1)
-  POTAMD_BATCH_CASE(16, 4, 2)
-  POTAMD_BATCH_CASE(16, 8, 2)
+  POTAMD_BATCH_CASE(4, 4, 1, 0)
+  POTAMD_BATCH_CASE(8, 4, 1, 0)
+  POTAMD_BATCH_CASE(16, 4, 1, 0)
+  POTAMD_BATCH_CASE(4, 8, 1, 0)
+  POTAMD_BATCH_CASE(8, 8, 1, 0)
+  POTAMD_BATCH_CASE(16, 8, 1, 0)
+  POTAMD_BATCH_CASE(16, 4, 2, 0)
+  POTAMD_BATCH_CASE(16, 8, 2, 0)
+  POTAMD_BATCH_CASE(4, 8, 1, 4)
+  POTAMD_BATCH_CASE(8, 8, 1, 4)
+  POTAMD_BATCH_CASE(16, 8, 1, 4)
 #undef POTAMD_BATCH_CASE
   return hipErrorInvalidConfiguration;
 }

@@ -424,14 +424,21 @@ extern "C" int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int ma
   ba.err_first = err_first;
   const int vec_ok = vec_aligned && ((uintptr_t)M % 16 == 0);
   // waves: low byte = wavefronts per problem (4 or 8), next byte = rows
-  // prefetched per wave (1 or 2; 2 only with 1024-column tiles).
+  // prefetched per wave (1 or 2; 2 only with 1024-column tiles), third byte =
+  // register cap in waves per SIMD (0 = library default, 4 = two 8-wave
+  // workgroups per CU, 255 = none).
   int pf = (waves >> 8) & 0xff;
+  int occ = (waves >> 16) & 0xff;
   waves &= 0xff;
   if (waves == 0) waves = batch_waves_default();
   if (pf == 0) pf = 1;
-  if ((waves != 4 && waves != 8) || (pf != 1 && pf != 2) || (pf == 2 && cpt != 16))
+  if (occ == 0) occ = batch_occupancy_default(waves, pf);
+  if (occ == 255) occ = 0;
+  if ((waves != 4 && waves != 8) || (pf != 1 && pf != 2) || (pf == 2 && cpt != 16) ||
+      (occ != 0 && occ != 4) || (occ == 4 && (waves != 8 || pf != 1)))
     return POTAMD_EINVAL;
-  HIP_TRY(launch_batch_log(cpt, vec_ok, waves, pf, ba, batch, static_cast<hipStream_t>(stream)));
+  HIP_TRY(launch_batch_log(cpt, vec_ok, waves, pf, occ, ba, batch,
+                           static_cast<hipStream_t>(stream)));
   return 0;
 }
 

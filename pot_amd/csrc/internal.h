@@ -107,8 +107,9 @@ struct BatchArgs {
 
 int batch_cpt(int max_m);
 int batch_waves_default();
-hipError_t launch_batch_log(int cpt, int vec_ok, int waves, int pf, const BatchArgs& a, int B,
-                            hipStream_t s);
+int batch_occupancy_default(int waves, int pf);
+hipError_t launch_batch_log(int cpt, int vec_ok, int waves, int pf, int occ,
+                            const BatchArgs& a, int B, hipStream_t s);
 
 struct RaggedDistArgs {
   const float* X;        // packed points, problem p rows at xoff[p]

@@ -74,6 +74,11 @@ __device__ __forceinline__ float bf16_val(unsigned short h) {
   return __uint_as_float((unsigned int)h << 16);
 }
 
+// A point coordinate as double; points are fp32 or fp64 (x64) in HBM.
+__device__ __forceinline__ double coord(const void* p, int64_t idx, int x64) {
+  return x64 ? static_cast<const double*>(p)[idx] : (double)static_cast<const float*>(p)[idx];
+}
+
 __device__ __forceinline__ double s2_of(const double* cscale, double reg) {
   return kLog2e / (reg * cscale[0]);
 }
@@ -84,7 +89,8 @@ __device__ __forceinline__ double s2_of(const double* cscale, double reg) {
 // precisions, the point scaled by sqrt(2 s2) (or unscaled) in the operand
 // format.  Points and coordinates beyond (rows, d) are zero.
 struct PackArgs {
-  const float* X;
+  const void* X;
+  int x64;
   int64_t ldx;
   int rows, d, D, rows_pad;
   const double* cscale;
@@ -105,7 +111,7 @@ __global__ __launch_bounds__(256) void online_pack_kernel(PackArgs A) {
   double nrm = 0.0;
   if (row < A.rows)
     for (int e = lane; e < A.d; e += 64) {
-      const double x = A.X[(int64_t)row * A.ldx + e];
+      const double x = coord(A.X, (int64_t)row * A.ldx + e, A.x64);
       nrm = fma(x, x, nrm);
     }
   nrm = wave_sum(nrm);
@@ -117,7 +123,7 @@ __global__ __launch_bounds__(256) void online_pack_kernel(PackArgs A) {
   for (int j = 0; j < 4; ++j) {
     const int e = e0 + j;
     float x = 0.f;
-    if (row < A.rows && e < A.d) x = A.X[(int64_t)row * A.ldx + e];
+    if (row < A.rows && e < A.d) x = (float)coord(A.X, (int64_t)row * A.ldx + e, A.x64);
     t[j] = (float)((double)x * sc);
   }
   if constexpr (P == POTAMD_ONLINE_F32) {
@@ -445,10 +451,11 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
 // fp64 pass (reference precision): 64 R points x 64 C points per stage,
 // 4 x 4 per thread, exact online maximum per point.
 struct F64PassArgs {
-  const float* R;
+  const void* R;
   int64_t ldr;
-  const float* C;
+  const void* C;
   int64_t ldc;
+  int x64;
   int nr, nc, d;
   const double* c_pot;
   const double* c_norm;
@@ -494,8 +501,8 @@ __global__ __launch_bounds__(256) void online_f64_pass_kernel(F64PassArgs A) {
         const int e = tid + 256 * q;
         const int row = e >> 5, col = e & 31;
         const int k = k0 + col;
-        Rs[row][col] = (r0 + row < A.nr && k < A.d) ? (double)A.R[(int64_t)(r0 + row) * A.ldr + k] : 0.0;
-        Cs[row][col] = (c0 + row < A.nc && k < A.d) ? (double)A.C[(int64_t)(c0 + row) * A.ldc + k] : 0.0;
+        Rs[row][col] = (r0 + row < A.nr && k < A.d) ? coord(A.R, (int64_t)(r0 + row) * A.ldr + k, A.x64) : 0.0;
+        Cs[row][col] = (c0 + row < A.nc && k < A.d) ? coord(A.C, (int64_t)(c0 + row) * A.ldc + k, A.x64) : 0.0;
       }
       if (k0 == 0 && tid < 64) {
         const int c = c0 + tid;
@@ -561,11 +568,12 @@ struct MaxReduceArgs {
   const double* part;
   const int* part_idx;
   int S, nr, nr_pad;
-  const float* X;
+  const void* X;
   int64_t ldx;
-  const float* Y;
+  const void* Y;
   int64_t ldy;
   int d;
+  int x64;
   double* blockmax;
   Ctrl* ctrl;
   double* cscale;
@@ -587,7 +595,8 @@ __global__ __launch_bounds__(256) void online_max_reduce_kernel(MaxReduceArgs A)
     }
     double acc = 0.0;
     for (int k = 0; k < A.d; ++k) {
-      const double df = (double)A.X[(int64_t)r * A.ldx + k] - (double)A.Y[(int64_t)bc * A.ldy + k];
+      const double df = coord(A.X, (int64_t)r * A.ldx + k, A.x64) -
+                        coord(A.Y, (int64_t)bc * A.ldy + k, A.x64);
       acc = fma(df, df, acc);
     }
     v = acc;
@@ -629,6 +638,8 @@ struct OnlineMergeArgs {
   const double* cscale;
   double reg;
   int check, decide, iter, max_iter;
+  int l1;         // error: 1 = sum |marginal - w| (empirical_sinkhorn), 0 = 2-norm
+  int inclusive;  // stop: 1 = err <= stop_thr (empirical_sinkhorn), 0 = err < stop_thr
   double stop_thr;
   double* err_hist;
   int err_cap;
@@ -656,7 +667,7 @@ __global__ __launch_bounds__(256) void online_merge_kernel(OnlineMergeArgs A) {
     const double wr = A.w[r];
     if (A.check) {
       const double cs = exp2(A.pot_cur[r] + L2);
-      e2 = (cs - wr) * (cs - wr);
+      e2 = A.l1 ? fabs(cs - wr) : (cs - wr) * (cs - wr);
     }
     A.pot_new[r] = wr > 0.0 ? A.logw2[r] - L2 : -__builtin_inf();
     A.shift[r] = I;
@@ -679,12 +690,12 @@ __global__ __launch_bounds__(256) void online_merge_kernel(OnlineMergeArgs A) {
   if (A.check) {
     double acc = 0.0;
     for (int b = 0; b < (int)gridDim.x; ++b) acc += A.errpart[b];
-    const double err = sqrt(acc);
+    const double err = A.l1 ? acc : sqrt(acc);
     ctrl->err = err;
     const int k = ctrl->n_err;
     if (k < A.err_cap) A.err_hist[k] = err;
     ctrl->n_err = k + 1;
-    if (err < A.stop_thr) {
+    if (A.inclusive ? err <= A.stop_thr : err < A.stop_thr) {
       ctrl->done = 1;
       ctrl->status = kConverged;
       ctrl->niter = it;
@@ -703,10 +714,11 @@ __global__ __launch_bounds__(256) void online_merge_kernel(OnlineMergeArgs A) {
 // P_rc = exp(-M_rc / reg + u_r + v_c).  One workgroup per 64 rows sweeps all
 // columns; column sums are fp64 atomics, row sums and cost are per block.
 struct OnlinePlanArgs {
-  const float* X;
+  const void* X;
   int64_t ldx;
-  const float* Y;
+  const void* Y;
   int64_t ldy;
+  int x64;
   int n, m, d;
   double reg;
   const double* cscale;
@@ -751,8 +763,8 @@ __global__ __launch_bounds__(256) void online_plan_kernel(OnlinePlanArgs A) {
         const int e = tid + 256 * q;
         const int row = e >> 5, col = e & 31;
         const int k = k0 + col;
-        Rs[row][col] = (r0 + row < A.n && k < A.d) ? (double)A.X[(int64_t)(r0 + row) * A.ldx + k] : 0.0;
-        Cs[row][col] = (c0 + row < A.m && k < A.d) ? (double)A.Y[(int64_t)(c0 + row) * A.ldy + k] : 0.0;
+        Rs[row][col] = (r0 + row < A.n && k < A.d) ? coord(A.X, (int64_t)(r0 + row) * A.ldx + k, A.x64) : 0.0;
+        Cs[row][col] = (c0 + row < A.m && k < A.d) ? coord(A.Y, (int64_t)(c0 + row) * A.ldy + k, A.x64) : 0.0;
       }
       __syncthreads();
 #pragma unroll 8
@@ -980,7 +992,8 @@ hipError_t launch_pass(int prec, int D, const PassArgs& a, int S, hipStream_t s)
 }
 
 struct Side {
-  const float* P;   // fp32 points
+  const void* P;    // points (fp32, or fp64 when x64)
+  int x64;
   int64_t ld;
   int count, pad;
   char* pk;
@@ -1001,6 +1014,7 @@ hipError_t half_step(int prec, int D, int d, const Side& R, const Side& C, const
   const int Seff = (nsteps + sps - 1) / sps;
   if (prec == POTAMD_ONLINE_F64) {
     F64PassArgs fa;
+    fa.x64 = R.x64;
     fa.R = R.P;
     fa.ldr = R.ld;
     fa.C = C.P;
@@ -1075,18 +1089,23 @@ extern "C" size_t potamd_online_workspace_size(int precision, int n, int m, int 
   return carve_online(nullptr, precision, n, m, d).bytes;
 }
 
-extern "C" int potamd_sinkhorn_online(int precision, const float* X, int64_t ldx, const float* Y,
-                                      int64_t ldy, int n, int m, int d, double reg,
+extern "C" int potamd_sinkhorn_online(int precision, int dtype, const void* X, int64_t ldx,
+                                      const void* Y, int64_t ldy, int n, int m, int d, double reg,
                                       double cost_scale, const double* a, const double* b,
                                       double* u, double* v, int max_iter, double stop_thr,
-                                      double* err_hist, int err_cap, int splits, void* workspace,
+                                      int semantics, double* err_hist, int err_cap, int splits,
+                                      void* workspace,
                                       size_t ws_bytes, void* host_ctrl, void* stream, int* info,
                                       double* cost_scale_out) {
   if (!X || !Y || !a || !b || !u || !v || !info || n < 1 || m < 1 || d < 1 || ldx < d ||
       ldy < d || !(reg > 0.0) || cost_scale < 0.0 || max_iter < 1 || splits < 0 || splits > 64 ||
-      (err_cap > 0 && !err_hist))
+      (semantics != 0 && semantics != 1) || (err_cap > 0 && !err_hist) ||
+      (dtype != POTAMD_F32 && dtype != POTAMD_F64))
     return POTAMD_EINVAL;
   if (!supported(precision, d)) return POTAMD_EUNSUPPORTED;
+  // fp64 points feed the fp64 pass only (the MFMA operands are packed from fp32)
+  if (dtype == POTAMD_F64 && precision != POTAMD_ONLINE_F64) return POTAMD_EUNSUPPORTED;
+  const int x64 = dtype == POTAMD_F64;
   const int prec = precision;
   const int D = padded_dim(prec, d);
   OnlineLayout L = carve_online(workspace, prec, n, m, d);
@@ -1102,13 +1121,14 @@ extern "C" int potamd_sinkhorn_online(int precision, const float* X, int64_t ldx
   HIP_TRY(fill(np, nan, L.shX, s));
   HIP_TRY(fill(mp, nan, L.shY, s));
 
-  Side SX{X, ldx, n, np, L.pkX, L.loX, L.scX, L.xn, L.shX, a, L.loga2};
-  Side SY{Y, ldy, m, mp, L.pkY, L.loY, L.scY, L.yn, L.shY, b, L.logb2};
+  Side SX{X, x64, ldx, n, np, L.pkX, L.loX, L.scX, L.xn, L.shX, a, L.loga2};
+  Side SY{Y, x64, ldy, m, mp, L.pkY, L.loY, L.scY, L.yn, L.shY, b, L.logb2};
 
   PackArgs pk;
   memset(&pk, 0, sizeof(pk));
   pk.cscale = L.cscale;
   pk.reg = reg;
+  pk.x64 = x64;
   auto pack_side = [&](int p, const Side& S, int scaled) {
     pk.X = S.P;
     pk.ldx = S.ld;
@@ -1164,6 +1184,7 @@ extern "C" int potamd_sinkhorn_online(int precision, const float* X, int64_t ldx
     mr.S = Seff;
     mr.nr = n;
     mr.nr_pad = np;
+    mr.x64 = x64;
     mr.X = X;
     mr.ldx = ldx;
     mr.Y = Y;
@@ -1188,6 +1209,8 @@ extern "C" int potamd_sinkhorn_online(int precision, const float* X, int64_t ldx
   ma.stop_thr = stop_thr;
   ma.err_hist = err_hist;
   ma.err_cap = err_cap;
+  ma.l1 = semantics;
+  ma.inclusive = semantics;
 
   // Prologue: v_1 = log b - LSE_i(Mr + u_0).
   ma.check = 0;
@@ -1217,8 +1240,10 @@ extern "C" int potamd_sinkhorn_online(int precision, const float* X, int64_t ldx
       rc = POTAMD_EHIP;
       break;
     }
-    // v update from the new u; its column sums give the check of iteration ii.
-    const bool check = ii % 10 == 0;
+    // v update from the new u; its column sums give the check of iteration ii
+    // (sinkhorn_log checks at ii % 10 == 0, empirical_sinkhorn at
+    // (ii + 1) % 10 == 0).
+    const bool check = semantics ? (ii + 1) % 10 == 0 : ii % 10 == 0;
     ma.check = check;
     ma.decide = check || ii == max_iter - 1;
     ma.pot_cur = L.V[(ii + 1) & 1];
@@ -1289,14 +1314,16 @@ extern "C" size_t potamd_online_plan_workspace_size(int n, int m) {
   return c.off;
 }
 
-extern "C" int potamd_online_plan(const float* X, int64_t ldx, const float* Y, int64_t ldy, int n,
+extern "C" int potamd_online_plan(int dtype, const void* X, int64_t ldx, const void* Y, int64_t ldy, int n,
                                   int m, int d, double reg, double cost_scale, const double* u,
                                   const double* v, const double* a, const double* b, double* P,
                                   int64_t ldp, double* sums, void* workspace, size_t ws_bytes,
                                   void* stream) {
   if (!X || !Y || !u || !v || !a || !b || !sums || n < 1 || m < 1 || d < 1 || ldx < d ||
-      ldy < d || !(reg > 0.0) || !(cost_scale > 0.0) || (P && ldp < m))
+      ldy < d || !(reg > 0.0) || !(cost_scale > 0.0) || (P && ldp < m) ||
+      (dtype != POTAMD_F32 && dtype != POTAMD_F64))
     return POTAMD_EINVAL;
+  const int x64 = dtype == POTAMD_F64;
   Carver c(workspace);
   double* xn = c.take<double>(n);
   double* yn = c.take<double>(m);
@@ -1312,6 +1339,7 @@ extern "C" int potamd_online_plan(const float* X, int64_t ldx, const float* Y, i
   memset(&pk, 0, sizeof(pk));
   pk.d = d;
   pk.D = d;
+  pk.x64 = x64;
   pk.X = X;
   pk.ldx = ldx;
   pk.rows = n;
@@ -1325,6 +1353,7 @@ extern "C" int potamd_online_plan(const float* X, int64_t ldx, const float* Y, i
   pk.norms = yn;
   HIP_TRY(launch_pack(POTAMD_ONLINE_F64, pk, s));
   OnlinePlanArgs pa;
+  pa.x64 = x64;
   pa.X = X;
   pa.ldx = ldx;
   pa.Y = Y;

@@ -32,11 +32,11 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
-def _points(X, dev):
+def _points(X, dev, dtype=torch.float32):
     X = torch.as_tensor(X)
     if X.device != dev:
         X = X.to(dev)
-    X = X.to(torch.float32)
+    X = X.to(dtype)
     if X.dim() != 2:
         raise ValueError("point clouds must be 2-D (n, d)")
     if X.stride(-1) != 1:
@@ -55,7 +55,7 @@ def _weights(w, k, dev):
 
 def sinkhorn_online(X, Y, reg, a=None, b=None, numItermax=1000, stopThr=1e-9, *,
                     precision="f32", cost_scale=None, warmstart=None, log=False, warn=True,
-                    splits=0, device=None):
+                    splits=0, device=None, _semantics=0):
     """Online-cost ``sinkhorn_log``; returns ``(log_u, log_v, log_dict)``.
 
     ``X`` (n, d) and ``Y`` (m, d) are point clouds (NumPy or torch; computed
@@ -69,8 +69,10 @@ def sinkhorn_online(X, Y, reg, a=None, b=None, numItermax=1000, stopThr=1e-9, *,
                          f"{sorted(_lib.ONLINE_PRECISIONS)}")
     dev = _device.require_gpu() if device is None else torch.device(device)
     lib = _lib.load()
-    Xt = _points(X, dev)
-    Yt = _points(Y, dev)
+    # fp64 points keep their precision on the fp64 pass; the MFMA passes pack fp32
+    pdt = torch.float64 if precision == "f64" else torch.float32
+    Xt = _points(X, dev, pdt)
+    Yt = _points(Y, dev, pdt)
     n, d = Xt.shape
     m = Yt.shape[0]
     if Yt.shape[1] != d:
@@ -93,9 +95,11 @@ def sinkhorn_online(X, Y, reg, a=None, b=None, numItermax=1000, stopThr=1e-9, *,
     scale_out = ctypes.c_double(0.0)
     hc = _device.host_ctrl()
     rc = lib.potamd_sinkhorn_online(
-        prec, _ptr(Xt), Xt.stride(0), _ptr(Yt), Yt.stride(0), n, m, d, float(reg),
+        prec, _lib.F64 if pdt == torch.float64 else _lib.F32, _ptr(Xt), Xt.stride(0), _ptr(Yt),
+        Yt.stride(0), n, m, d, float(reg),
         0.0 if cost_scale is None else float(cost_scale), _ptr(at), _ptr(bt), _ptr(u), _ptr(v),
-        int(numItermax), float(stopThr), _ptr(err_hist), err_cap, int(splits), _ptr(ws),
+        int(numItermax), float(stopThr), int(_semantics), _ptr(err_hist), err_cap, int(splits),
+        _ptr(ws),
         ws_bytes, ctypes.c_void_p(hc.data_ptr()), ctypes.c_void_p(_device.stream_ptr(dev)), info,
         ctypes.byref(scale_out))
     _lib.check(rc, "potamd_sinkhorn_online")
@@ -111,17 +115,20 @@ def sinkhorn_online(X, Y, reg, a=None, b=None, numItermax=1000, stopThr=1e-9, *,
 
 
 def online_plan(X, Y, reg, log_u, log_v, cost_scale, a=None, b=None, *, want_plan=False,
-                device=None):
+                device=None, fp64_points=False):
     """fp64 plan / cost / marginal errors of the online cost.
 
+    ``fp64_points`` evaluates the cost from float64 coordinates (otherwise the
+    points are taken as float32).
     Returns ``(P or None, {"cost", "mass", "err_a", "err_b"})`` with
     ``P_ij = exp(-M_ij / reg + log_u_i + log_v_j)`` (ot/bregman/_sinkhorn.py:915-918)
     and ``cost = <P, M>`` (the loss of sinkhorn2, :436).
     """
     dev = _device.require_gpu() if device is None else torch.device(device)
     lib = _lib.load()
-    Xt = _points(X, dev)
-    Yt = _points(Y, dev)
+    pdt = torch.float64 if fp64_points else torch.float32
+    Xt = _points(X, dev, pdt)
+    Yt = _points(Y, dev, pdt)
     n, d = Xt.shape
     m = Yt.shape[0]
     at = _weights(a, n, dev)
@@ -133,7 +140,8 @@ def online_plan(X, Y, reg, log_u, log_v, cost_scale, a=None, b=None, *, want_pla
     ws_bytes = lib.potamd_online_plan_workspace_size(n, m)
     ws = _device.workspace(ws_bytes, dev)
     rc = lib.potamd_online_plan(
-        _ptr(Xt), Xt.stride(0), _ptr(Yt), Yt.stride(0), n, m, d, float(reg), float(cost_scale),
+        _lib.F64 if fp64_points else _lib.F32, _ptr(Xt), Xt.stride(0), _ptr(Yt), Yt.stride(0),
+        n, m, d, float(reg), float(cost_scale),
         _ptr(u), _ptr(v), _ptr(at), _ptr(bt), _ptr(P), m, _ptr(sums), _ptr(ws), ws_bytes,
         ctypes.c_void_p(_device.stream_ptr(dev)))
     _lib.check(rc, "potamd_online_plan")

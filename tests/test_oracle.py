@@ -276,3 +276,58 @@ def test_dist_bruteforce():
     Db = ob.dist_batch(x[None], y[None])[0]
     np.testing.assert_allclose(Db, brute, atol=1e-13)
     np.testing.assert_allclose(ou.dist(x, y, "euclidean"), np.sqrt(brute), atol=1e-13)
+
+
+def test_solve_entropic_value_identities():
+    # value formulas of ot/solvers.py:476-486 against the exact identities
+    # value_KL = reg (sum (u - log a) r + sum (v - log b) c),
+    # value_ent = reg (sum u r + sum v c), r, c the plan's marginals.
+    from oracle.solvers import solve_entropic
+    rng = np.random.default_rng(4)
+    n, m, reg = 30, 25, 0.2
+    M = rng.random((n, m))
+    a = rng.dirichlet(np.ones(n))
+    b = rng.dirichlet(np.ones(m))
+    for reg_type in ("KL", "entropy"):
+        P, (u, v), val, vl, _ = solve_entropic(M, a, b, reg, reg_type=reg_type, max_iter=37,
+                                              tol=0.0)
+        r, c = P.sum(1), P.sum(0)
+        assert abs(vl - np.sum(M * P)) < 1e-14
+        if reg_type == "KL":
+            ident = reg * (np.sum((u - np.log(a)) * r) + np.sum((v - np.log(b)) * c))
+        else:
+            ident = reg * (np.sum(u * r) + np.sum(v * c))
+        assert abs(val - ident) < 1e-9 * max(1.0, abs(ident))
+
+
+def test_solve_entropic_last_step_is_one_more_update():
+    from oracle.solvers import solve_entropic
+    rng = np.random.default_rng(5)
+    M = rng.random((12, 9))
+    P1, (u1, v1), val1, _, n1 = solve_entropic(M, reg=0.5, max_iter=8, tol=0.0,
+                                               grad="last_step")
+    P2, (u2, v2), val2, _, n2 = solve_entropic(M, reg=0.5, max_iter=8, tol=0.0)
+    np.testing.assert_allclose(P1, P2, rtol=1e-12)
+    np.testing.assert_allclose(val1, val2, rtol=1e-12)
+    assert n1 == 8 and n2 == 7
+
+
+def test_empirical_lazy_doctest_and_fixed_point():
+    # doctest of ot/bregman/_empirical.py:152-157 (reg = 0.1, two points)
+    from oracle.empirical import empirical_sinkhorn_lazy
+    X = np.arange(2, dtype=np.float64).reshape(2, 1)
+    f, g, log = empirical_sinkhorn_lazy(X, X, 0.1)
+    M = (X - X.T) ** 2
+    P = np.exp(f[:, None] + g[None, :] - M / 0.1)
+    np.testing.assert_allclose(P, [[4.99977301e-01, 2.26989344e-05],
+                                   [2.26989344e-05, 4.99977301e-01]], rtol=1e-8)
+    assert log["niter"] == 9 and len(log["err"]) == 1   # converges at the first check
+    rng = np.random.default_rng(0)
+    xs, xt = rng.standard_normal((20, 2)), rng.standard_normal((15, 2)) + 1
+    f, g, log = empirical_sinkhorn_lazy(xs, xt, 0.5, stopThr=1e-12)
+    M = ((xs[:, None] - xt[None]) ** 2).sum(-1)
+    from scipy.special import logsumexp
+    np.testing.assert_allclose(f, np.log(1 / 20) - logsumexp(g[None] - M / 0.5, 1), atol=1e-10)
+    np.testing.assert_allclose(g, np.log(1 / 15) - logsumexp(f[:, None] - M / 0.5, 0),
+                               atol=1e-12)
+    assert (log["niter"] + 1) % 10 == 0 and log["err"][-1] <= 1e-12
