@@ -47,6 +47,7 @@ def _args():
     ap.add_argument("--batch", type=int, default=4096, help="problems per step")
     ap.add_argument("--waves", type=int, default=0, help="wavefronts per problem (0 = default)")
     ap.add_argument("--prefetch", type=int, default=0, help="rows prefetched per wave (0 = 1)")
+    ap.add_argument("--rotate", type=int, default=0, help="batch kernel: rotate row order")
     ap.add_argument("--occ", type=int, default=0,
                     help="batch kernel register cap in waves/SIMD (0 default, 4, 255 none)")
     ap.add_argument("--check-every", type=int, default=1,
@@ -101,7 +102,7 @@ def run_warmstart(args, torch, dev):
                                              stopThr=args.stop, waves=args.waves,
                                              prefetch=args.prefetch,
                                              check_every=args.check_every,
-                                             occupancy=args.occ,
+                                             occupancy=args.occ, rotate=args.rotate,
                                              fallback_thr=ws_mod.FALLBACK_THR if warm else 0.0)
             e1.record()
             torch.cuda.synchronize()

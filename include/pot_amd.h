@@ -159,7 +159,8 @@ int potamd_sqeuclidean_ragged(int batch, int max_n, int max_m, int d, const floa
  * waves: wavefronts per problem (4 or 8; 0 = default) in the low byte, rows
  *    prefetched per wave (1 or 2, 0 = 1; 2 needs max_m > 512) in the next byte,
  *    register cap in waves per SIMD in the third (0 = default, 4 = two 8-wave
- *    workgroups per CU (8 waves, 1 row prefetched only), 255 = no cap).
+ *    workgroups per CU (8 waves, 1 row prefetched only), 255 = no cap); bit 24
+ *    starts each problem's row sweep at a problem-dependent row.
  */
 int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int max_m, const void* M,
                               const int64_t* moff, const int* ns, const int* ms, const int* ldms,

@@ -149,20 +149,26 @@ restart:
     float acc[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; ++q) acc[q] = 0.f;
+    // Row i of the sweep is physical row pr(i): with A.rotate, problem p
+    // starts its sweep at row rot, so equal-size problems streamed side by
+    // side do not hit the same address offsets at the same time.
+    const int rot = (A.rotate && n > 0) ? (int)(((int64_t)p * 337) % n) : 0;
+    auto pr = [&](int r) { const int q = r + rot; return q < n ? q : q - n; };
     int i = wid;
-    if (i < n) load_row(i, cur);
+    if (i < n) load_row(pr(i), cur);
     if constexpr (PF > 1) {
-      if (i + W < n) load_row(i + W, nxt);
+      if (i + W < n) load_row(pr(i + W), nxt);
     }
     for (; i < n; i += W) {
       if constexpr (PF > 1) {
-        if (i + 2 * W < n) load_row(i + 2 * W, nx2);
+        if (i + 2 * W < n) load_row(pr(i + 2 * W), nx2);
       } else {
-        if (i + W < n) load_row(i + W, nxt);
+        if (i + W < n) load_row(pr(i + W), nxt);
       }
-      const float a_i = af[i];
+      const int ip = pr(i);
+      const float a_i = af[ip];
       if (a_i > 0.f) {
-        double c2 = c2s[i];
+        double c2 = c2s[ip];
         float s = 0.f;
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
@@ -172,7 +178,7 @@ restart:
         }
         float S = wave_sum_dpp(s);
         if (!in_safe_range((double)S)) {
-          load_row(i, cur);
+          load_row(ip, cur);
           float mxr = -__builtin_inff();
 #pragma unroll
           for (int q = 0; q < CPT; ++q) {
@@ -190,9 +196,9 @@ restart:
             s += cur[q];
           }
           S = wave_sum_dpp(s);
-          if (lane == 0) c2s[i] = c2;  // the shift actually used
+          if (lane == 0) c2s[ip] = c2;  // the shift actually used
         }
-        if (lane == 0) sh[i] = S;
+        if (lane == 0) sh[ip] = S;
         const float kf = a_i * __builtin_amdgcn_rcpf(S);
 #pragma unroll
         for (int q = 0; q < CPT; ++q) acc[q] = __builtin_fmaf(cur[q], kf, acc[q]);

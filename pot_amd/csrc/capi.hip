@@ -426,9 +426,10 @@ extern "C" int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int ma
   // waves: low byte = wavefronts per problem (4 or 8), next byte = rows
   // prefetched per wave (1 or 2; 2 only with 1024-column tiles), third byte =
   // register cap in waves per SIMD (0 = library default, 4 = two 8-wave
-  // workgroups per CU, 255 = none).
+  // workgroups per CU, 255 = none), bit 24 = rotate each problem's row order.
   int pf = (waves >> 8) & 0xff;
   int occ = (waves >> 16) & 0xff;
+  ba.rotate = (waves >> 24) & 1;
   waves &= 0xff;
   if (waves == 0) waves = batch_waves_default();
   if (pf == 0) pf = 1;

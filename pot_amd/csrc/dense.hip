@@ -121,8 +121,11 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
     return vsh[((q / VEC) * T + tid) * VEC + (q % VEC)];
   };
 
-  const int r0 = g * A.rpb;
-  const int r1 = min(n, r0 + A.rpb);
+  // Rows are dealt round-robin: workgroup g owns rows g, g + G, g + 2G, ...
+  // so at any moment the G workgroups stream neighbouring rows (one
+  // contiguous region) instead of G regions a row block apart.
+  const int nloc = g < n ? (n - g + G - 1) / G : 0;
+  auto row_of = [&](int k) { return g + k * G; };
 
   // Rows in flight: the current one plus PF prefetched (rotating registers).
   R cur[CPT], nxt[CPT], nx2[PF > 1 ? CPT : 1];
@@ -142,24 +145,25 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
       for (int c = 0; c < VEC; ++c) dst[k * VEC + c] = (R)tmp[c];
     }
   };
-  if (r0 < r1) load_row(r0, cur);
+  if (nloc > 0) load_row(row_of(0), cur);
   if constexpr (PF > 1) {
-    if (r0 + 1 < r1) load_row(r0 + 1, nxt);
+    if (nloc > 1) load_row(row_of(1), nxt);
   }
 
   const R s_hi = (R)(sizeof(TM) == 4 ? (double)A.s_hi : A.s2);
   const R s_lo = (R)(sizeof(TM) == 4 ? (double)A.s_lo : 0.0);
 
-  for (int i = r0; i < r1; ++i) {
+  for (int k = 0; k < nloc; ++k) {
+    const int i = row_of(k);
     if constexpr (PF > 1) {
-      if (i + 2 < r1) load_row(i + 2, nx2);
+      if (k + 2 < nloc) load_row(row_of(k + 2), nx2);
     } else {
-      if (i + 1 < r1) load_row(i + 1, nxt);
+      if (k + 1 < nloc) load_row(row_of(k + 1), nxt);
     }
     const double a_i = ap[i];
     if (DOMAIN == kLog) {
       if (a_i == 0.0) {
-        if (tid == 0) { row_c[i - r0] = __builtin_nan(""); row_s[i - r0] = 0.0; }
+        if (tid == 0) { row_c[k] = __builtin_nan(""); row_s[k] = 0.0; }
       } else {
         double c2 = la2[i] - uo[i];
         if (!__builtin_isfinite(c2)) c2 = 0.0;
@@ -191,7 +195,7 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
           }
           S = (double)block_sum<W>(s, red, parity);
         }
-        if (tid == 0) { row_c[i - r0] = c2; row_s[i - r0] = S; }
+        if (tid == 0) { row_c[k] = c2; row_s[k] = S; }
         R kf;
         if (sizeof(R) == 4)
           kf = (R)((float)a_i * __builtin_amdgcn_rcpf((float)S));
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
 #pragma unroll
         for (int q = 0; q < CPT; ++q) s = __builtin_fma(cur[q], (R)vcol(q), s);
         const double S = (double)block_sum<W>(s, red, parity);
-        if (tid == 0) row_s[i - r0] = S;
+        if (tid == 0) row_s[k] = S;
         R f;
         if (sizeof(R) == 4)
           f = (R)((float)a_i * __builtin_amdgcn_rcpf((float)S));
@@ -247,8 +251,8 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
   __syncthreads();
   double rerr = 0.0;
   int bad = 0;
-  for (int r = tid; r < r1 - r0; r += T) {
-    const int i = r0 + r;
+  for (int r = tid; r < nloc; r += T) {
+    const int i = row_of(r);
     const double a_i = ap[i];
     const double S = row_s[r];
     if (DOMAIN == kLog) {

@@ -96,6 +96,7 @@ struct BatchArgs {
   double* v;             // out: log_v
   const int* order;      // [B] launch order (largest first) or null
   int max_iter;
+  int rotate;            // start each problem's row sweep at a problem-dependent row
   int check_every;       // iterations between marginal checks (10 in the reference)
   double stop_thr;
   double fallback_thr;   // > 0: restart cold when the first check exceeds it

@@ -179,7 +179,8 @@ def launch_order(R, work=None):
 
 
 def sinkhorn_log_ragged(R, u0=None, numItermax=1000, stopThr=1e-9, order=None,
-                        fallback_thr=0.0, waves=0, prefetch=0, check_every=10, occupancy=0):
+                        fallback_thr=0.0, waves=0, prefetch=0, check_every=10, occupancy=0,
+                        rotate=0):
     """Solve every problem of ``R`` with sinkhorn_log semantics in one launch.
 
     u0: (sum n,) float64 initial log potentials (None = zeros, the cold start).
@@ -208,7 +209,7 @@ def sinkhorn_log_ragged(R, u0=None, numItermax=1000, stopThr=1e-9, order=None,
         _ptr(R.b), _ptr(u), _ptr(v), int(numItermax), int(check_every), float(stopThr),
         float(fallback_thr),
         _ptr(order), 1, _ptr(niter), _ptr(status), _ptr(e_last), _ptr(e_first),
-        int(waves) | (int(prefetch) << 8) | (int(occupancy) << 16),
+        int(waves) | (int(prefetch) << 8) | (int(occupancy) << 16) | (int(bool(rotate)) << 24),
         ctypes.c_void_p(_device.stream_ptr(dev)))
     _lib.check(rc, "potamd_sinkhorn_log_batch")
     return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,

@@ -62,7 +62,10 @@ def test_gemm_epilogues_and_split_k():
     out = torch.empty((M, N), dtype=torch.bfloat16, device=dev)
     net._gemm(A, B, out, M, N, K, bias=bias, rowvec=rowvec, seg_len=seg, relu=1, mask=mask)
     ref = A.float() @ B.float() + bias + rowvec.repeat_interleave(seg, 0)
-    ref = torch.relu(ref) * (mask.float() > 0)
+    # leaky ReLU (slope 0.01), then the backward mask: keep where mask > 0,
+    # scale by the slope elsewhere
+    ref = torch.where(ref > 0, ref, 0.01 * ref)
+    ref = torch.where(mask.float() > 0, ref, 0.01 * ref)
     torch.cuda.synchronize()
     np.testing.assert_allclose(out.float().cpu(), ref.cpu(), rtol=1e-2, atol=1e-2)
     # split-K accumulation into fp32 (weight-gradient form A^T B over many rows)
