@@ -1,0 +1,75 @@
+"""Float64 NumPy restatement of ``ot.batch``'s Gromov-Wasserstein solver.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Follows ot/batch/_quadratic.py: ``tensor_batch`` (:17-152, sqeuclidean loss
+f1 = C^2, f2 = C^2, h1 = 2C, h2 = C), ``tensor_product_batch`` (:535-572,
+``const - hC1 T hC2^T``, averaged with the transposed product when not
+symmetric), ``compute_const_from_marginals`` (:581-590) and
+``solve_gromov_batch`` (:269-501: T0 = a b^T / sqrt(|a||b|), each step
+``K = -((1 - alpha) M + 2 alpha L x T) / reg + log(T + 1e-15)`` projected by
+``bregman_log_projection_batch``, stop when the largest mean |T - T_prev|
+< tol; values recomputed with the plan's marginals).
+
+Pinned (tests/test_oracle.py) by the brute-force 4-D tensor
+``sum_jl (C1_ij - C2_kl)^2 T_jl`` on small problems, symmetric and not.
+"""
+
+import numpy as np
+
+from .batch import bregman_log_projection_batch
+
+
+def tensor_batch(a, b, C1, C2, symmetric=True):
+    fC1, fC2 = C1 ** 2, C2 ** 2
+    if not symmetric:
+        fC1 = 0.5 * (fC1 + fC1.transpose(0, 2, 1))
+        fC2 = 0.5 * (fC2 + fC2.transpose(0, 2, 1))
+    constC = np.einsum("bij,bj->bi", fC1, a)[:, :, None] + np.einsum("bij,bj->bi", fC2, b)[:, None, :]
+    return {"constC": constC, "hC1": 2 * C1, "hC2": C2, "fC1": fC1, "fC2": fC2}
+
+
+def tensor_product_batch(L, T, recompute_const=False, symmetric=True):
+    if recompute_const:
+        const = (np.einsum("bij,bj->bi", L["fC1"], T.sum(2))[:, :, None]
+                 + np.einsum("bij,bj->bi", L["fC2"], T.sum(1))[:, None, :])
+    else:
+        const = L["constC"]
+    dot = np.einsum("bij,bjk,blk->bil", L["hC1"], T, L["hC2"])
+    if not symmetric:
+        dot_t = np.einsum("bji,bjk,bkl->bil", L["hC1"], T, L["hC2"])
+        dot = (dot + dot_t) / 2
+    return const - dot
+
+
+def loss_quadratic_batch(L, T, recompute_const=False, symmetric=True):
+    return np.sum(tensor_product_batch(L, T, recompute_const, symmetric) * T, axis=(1, 2))
+
+
+def solve_gromov_batch(C1, C2, reg=1e-2, a=None, b=None, symmetric=None, M=None, alpha=None,
+                       max_iter=50, tol=1e-5, max_iter_inner=50, tol_inner=1e-5):
+    B, n, m = C1.shape[0], C1.shape[1], C2.shape[1]
+    a = np.ones((B, n)) / n if a is None else a
+    b = np.ones((B, m)) / m if b is None else b
+    if symmetric is None:
+        symmetric = (np.allclose(C1, C1.transpose(0, 2, 1), atol=1e-10)
+                     and np.allclose(C2, C2.transpose(0, 2, 1), atol=1e-10))
+    L = tensor_batch(a, b, C1, C2, symmetric)
+    if M is None:
+        M, alpha = np.zeros((B, n, m)), 1.0
+    const = np.sqrt(a.sum(1) * b.sum(1))
+    T = a[:, :, None] * b[:, None, :] / const[:, None, None]
+    log_T = np.log(T + 1e-15)
+    it = -1
+    for it in range(max_iter):
+        T_prev = T
+        Mk = (1 - alpha) * M + 2 * alpha * tensor_product_batch(L, T, symmetric=symmetric)
+        out = bregman_log_projection_batch(-Mk / reg + log_T, a, b, max_iter=max_iter_inner,
+                                           tol=tol_inner)
+        T, log_T = out["T"], out["log_T"]
+        if np.max(np.mean(np.abs(T_prev - T), axis=(1, 2))) < tol:
+            break
+    vl = np.sum(M * T, axis=(1, 2))
+    vq = loss_quadratic_batch(L, T, recompute_const=True, symmetric=symmetric)
+    return {"T": T, "value": (1 - alpha) * vl + alpha * vq, "value_linear": vl * (1 - alpha),
+            "value_quad": vq * alpha, "n_iter": it, "potentials": out["potentials"]}

@@ -194,3 +194,13 @@ def solve_sample_batch(X_a, X_b, reg, a=None, b=None, metric="sqeuclidean", p=2,
     M = dist_batch(X_a, X_b, metric=metric, p=p)
     return solve_batch(M, reg, a=a, b=b, max_iter=max_iter, tol=tol, solver=solver,
                        reg_type=reg_type, grad=grad)
+
+
+# Quadratic (Gromov-Wasserstein) batch solvers share this namespace, as in
+# ot.batch (ot/batch/__init__.py).
+from .quadratic import (  # noqa: E402
+    loss_quadratic_batch,
+    loss_quadratic_samples_batch,
+    solve_gromov_batch,
+    tensor_batch,
+)

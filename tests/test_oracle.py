@@ -331,3 +331,29 @@ def test_empirical_lazy_doctest_and_fixed_point():
     np.testing.assert_allclose(g, np.log(1 / 15) - logsumexp(f[:, None] - M / 0.5, 0),
                                atol=1e-12)
     assert (log["niter"] + 1) % 10 == 0 and log["err"][-1] <= 1e-12
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_gw_tensor_product_bruteforce(symmetric):
+    # L x T of ot/batch/_quadratic.py against the 4-D tensor
+    # sum_jl l(C1_ij, C2_kl) T_jl (averaged with l(C1_ji, C2_lk) when the
+    # costs are not symmetric), l the squared difference.
+    from oracle.quadratic import tensor_batch, tensor_product_batch
+    rng = np.random.default_rng(7)
+    B, n, m = 2, 5, 4
+    C1 = rng.random((B, n, n))
+    C2 = rng.random((B, m, m))
+    if symmetric:
+        C1 = C1 + C1.transpose(0, 2, 1)
+        C2 = C2 + C2.transpose(0, 2, 1)
+    T = rng.random((B, n, m))
+    T /= T.sum(axis=(1, 2), keepdims=True)
+    L = tensor_batch(T.sum(2), T.sum(1), C1, C2, symmetric=symmetric)
+    got = tensor_product_batch(L, T, symmetric=symmetric)
+    loss = (C1[:, :, :, None, None] - C2[:, None, None, :, :]) ** 2   # b i j k l
+    ref = np.einsum("bijkl,bjl->bik", loss, T)
+    if not symmetric:
+        loss_t = (C1.transpose(0, 2, 1)[:, :, :, None, None]
+                  - C2.transpose(0, 2, 1)[:, None, None, :, :]) ** 2
+        ref = 0.5 * (ref + np.einsum("bijkl,bjl->bik", loss_t, T))
+    np.testing.assert_allclose(got, ref, rtol=1e-12)
