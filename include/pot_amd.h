@@ -254,6 +254,30 @@ int potamd_online_plan(int dtype, const void* X, int64_t ldx, const void* Y, int
                        const double* a, const double* b, double* P, int64_t ldp, double* sums,
                        void* workspace, size_t ws_bytes, void* stream);
 
+/* ---- Log-stabilised Sinkhorn ------------------------------------------ */
+
+/* Workspace of potamd_sinkhorn_stabilized (holds K in the cost's dtype). */
+size_t potamd_stabilized_workspace_size(int dtype, int n, int m);
+
+/* ot.bregman.sinkhorn_stabilized (ot/bregman/_sinkhorn.py:1105-1367), one
+ * histogram: u, v start at 1/dim; v = b / K^T u, u = a / K v with
+ * K = exp(-(M - alpha - beta) / reg); when max|u| or max|v| exceeds tau,
+ * alpha += reg log u, beta += reg log v, u, v restart at 1/dim and K is
+ * rebuilt; every print_period-th iteration err = ||Gamma^T 1 - b||_2
+ * (err_hist); err <= stop_thr stops; a NaN restores the previous u, v and
+ * returns POTAMD_NUMERICAL.  alpha, beta (fp64 device, in: warm start) hold
+ * the absorbed potentials on exit and u, v (fp64 device, out) the scalings,
+ * so log u = alpha / reg + log(u) as in the reference's log.  marg_sq (device,
+ * 2 doubles, may be NULL) receives ||Gamma^T 1 - b||^2 and ||Gamma 1 - a||^2
+ * of the final plan (the check of sinkhorn_epsilon_scaling, :1502-1505).
+ * info as potamd_sinkhorn_dense. */
+int potamd_sinkhorn_stabilized(int dtype, const void* M, int64_t ldm, int n, int m, double reg,
+                               const double* a, const double* b, double* alpha, double* beta,
+                               double* u, double* v, double tau, int max_iter, double stop_thr,
+                               int print_period, double* err_hist, int err_cap, double* marg_sq,
+                               void* workspace, size_t ws_bytes, void* host_ctrl, void* stream,
+                               int* info);
+
 #ifdef __cplusplus
 }
 #endif

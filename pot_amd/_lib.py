@@ -64,6 +64,13 @@ SIGNATURES = {
     "potamd_online_plan": (c_int, [
         c_int, c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int, c_dbl, c_dbl,
         c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_size, c_vp]),
+    "potamd_stabilized_workspace_size": (c_size, [c_int, c_int, c_int]),
+    "potamd_sinkhorn_stabilized": (c_int, [
+        c_int, c_vp, c_i64, c_int, c_int, c_dbl,   # dtype, M, ldm, n, m, reg
+        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,        # a, b, alpha, beta, u, v
+        c_dbl, c_int, c_dbl, c_int,                # tau, max_iter, stop_thr, print_period
+        c_vp, c_int, c_vp,                         # err_hist, err_cap, marg_sq
+        c_vp, c_size, c_vp, c_vp, c_ip]),          # workspace, ws_bytes, host_ctrl, stream, info
     "potamd_sqeuclidean": (c_int, [
         c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_i64, c_i64, c_vp, c_i64,
         c_i64, c_int, c_int, c_int, c_vp]),

@@ -27,7 +27,7 @@ NOT_CONVERGED = (
     "increase the number of iterations `numItermax` "
     "or the regularization parameter `reg`."
 )
-_UNBUILT = ("greenkhorn", "sinkhorn_stabilized", "sinkhorn_epsilon_scaling")
+_UNBUILT = ("greenkhorn",)
 
 
 def _print_errs(errs, start=0, step=10):
@@ -60,6 +60,16 @@ def sinkhorn(a, b, M, reg, method="sinkhorn", numItermax=1000, stopThr=1e-9,
         return sinkhorn_log(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
                             verbose=verbose, log=log, warn=warn,
                             warmstart=warmstart, **kwargs)
+    if name == "sinkhorn_stabilized":
+        from .stabilized import sinkhorn_stabilized
+        return sinkhorn_stabilized(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
+                                   warmstart=warmstart, verbose=verbose, log=log, warn=warn,
+                                   **kwargs)
+    if name == "sinkhorn_epsilon_scaling":
+        from .stabilized import sinkhorn_epsilon_scaling
+        return sinkhorn_epsilon_scaling(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
+                                        warmstart=warmstart, verbose=verbose, log=log,
+                                        warn=warn, **kwargs)
     raise _method_error(method)
 
 
@@ -68,6 +78,14 @@ def sinkhorn2(a, b, M, reg, method="sinkhorn", numItermax=1000, stopThr=1e-9,
     """Entropic OT loss ``<P, M>`` (_sinkhorn.py:387-484); ``warn`` defaults to False."""
     M, a, b = list_to_array(M, a, b)
     name = method.lower()
+    if name == "sinkhorn_stabilized":
+        if len(b.shape) >= 2:
+            raise NotImplementedError("sinkhorn_stabilized with several histograms is not "
+                                      "part of this build")
+        from .stabilized import sinkhorn_stabilized
+        return sinkhorn_stabilized(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
+                                   warmstart=warmstart, verbose=verbose, log=log, warn=warn,
+                                   _loss_only=True, **kwargs)
     if name not in ("sinkhorn", "sinkhorn_log"):
         raise _method_error(method)
     solver = sinkhorn_knopp if name == "sinkhorn" else sinkhorn_log
@@ -239,3 +257,7 @@ def sinkhorn_knopp(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
                 "u": ing.out(uu), "v": ing.out(vv)}
         return out, logd
     return out
+
+
+# ot.bregman also exposes the stabilised solvers (ot/bregman/_sinkhorn.py:1105, :1371)
+from .stabilized import sinkhorn_epsilon_scaling, sinkhorn_stabilized  # noqa: E402

@@ -357,3 +357,29 @@ def test_gw_tensor_product_bruteforce(symmetric):
                   - C2.transpose(0, 2, 1)[:, None, None, :, :]) ** 2
         ref = 0.5 * (ref + np.einsum("bijkl,bjl->bik", loss_t, T))
     np.testing.assert_allclose(got, ref, rtol=1e-12)
+
+
+def test_stabilized_and_eps_scaling_oracle():
+    # doctests _sinkhorn.py:1191-1193 and :1458-1460
+    from oracle.stabilized import sinkhorn_epsilon_scaling, sinkhorn_stabilized
+    want = [[0.36552929, 0.13447071], [0.13447071, 0.36552929]]
+    np.testing.assert_allclose(sinkhorn_stabilized([.5, .5], [.5, .5], [[0., 1.], [1., 0.]], 1),
+                               want, atol=5e-9)
+    np.testing.assert_allclose(
+        sinkhorn_epsilon_scaling([.5, .5], [.5, .5], [[0., 1.], [1., 0.]], 1), want, atol=5e-9)
+    # stabilized == sinkhorn_log at convergence (test_bregman.py:282-314), with
+    # the default tau and with tau = 100; both absorb during the early
+    # iterations.  (A tau so small that every iteration absorbs never
+    # converges: u, v restart at 1/dim after an absorption.)
+    rng = np.random.default_rng(8)
+    n, m = 40, 30
+    M = rng.random((n, m))
+    a, b = rng.dirichlet(np.ones(n)), rng.dirichlet(np.ones(m))
+    P_log = osk.sinkhorn_log(a, b, M, 0.05, stopThr=1e-12, numItermax=5000)
+    P1, l1 = sinkhorn_stabilized(a, b, M, 0.05, stopThr=1e-12, numItermax=5000, log=True)
+    P2, l2 = sinkhorn_stabilized(a, b, M, 0.05, stopThr=1e-12, numItermax=5000, tau=100.0,
+                                 log=True)
+    np.testing.assert_allclose(P1, P_log, atol=1e-10)
+    np.testing.assert_allclose(P2, P_log, atol=1e-10)
+    assert l1["n_iter"] % 20 == 0 and l1["err"][-1] <= 1e-12
+    assert np.abs(l2["alpha"]).max() > 0  # absorbed at least once
