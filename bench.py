@@ -30,7 +30,7 @@ import numpy as np
 
 METRIC = "problems/sec to marginal err≤1e-6 (learned warm-start vs plain GPU Sinkhorn)"
 HBM_PEAK_GBS = 8000.0
-MFMA_PEAK_TFLOPS = {"f32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0, "fp8": 5000.0, "f64": 78.6}
+MFMA_PEAK_TFLOPS = {"f32": 157.3, "f16": 2500.0, "bf16": 2500.0, "bf16x3": 2500.0, "fp8": 5000.0, "f64": 78.6}
 
 
 def _args():
@@ -55,6 +55,8 @@ def _args():
                          "with err <= stopThr; 10: the reference's cadence)")
     ap.add_argument("--stop", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=2000)
+    ap.add_argument("--cost", default="dense", choices=["dense", "points"],
+                    help="warmstart: materialised cost (dense) or computed on the fly (points)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cold-only", action="store_true")
     return ap.parse_args()
@@ -89,21 +91,34 @@ def run_warmstart(args, torch, dev):
         t_total = 0.0
         kern_ms = 0.0
         alg_bytes = 0.0
+        elems = 0.0        # elements of the fused sweeps (one exp2 each)
+        sweeps_tot = 0.0   # elements of every pass over the cost
         iters, conv, fb = [], [], []
         for k in range(nsteps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            R = ragged.from_clouds(X, Y, regs)
-            u0 = ws_mod.predict_u0(net, R, X, Y, regs) if warm else None
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            res = ragged.sinkhorn_log_ragged(R, u0=u0, numItermax=args.max_iter,
-                                             stopThr=args.stop, waves=args.waves,
-                                             prefetch=args.prefetch,
-                                             check_every=args.check_every,
-                                             occupancy=args.occ, rotate=args.rotate,
-                                             fallback_thr=ws_mod.FALLBACK_THR if warm else 0.0)
+            fb_thr = ws_mod.FALLBACK_THR if warm else 0.0
+            if args.cost == "points":
+                R = ragged.points_from_clouds(X, Y, regs)
+                cm = ragged.points_cmax(R) if warm else None
+                u0 = ws_mod.predict_u0(net, R, X, Y, regs) if warm else None
+                e0.record()
+                res = ragged.sinkhorn_log_ragged_points(R, u0=u0, numItermax=args.max_iter,
+                                                        stopThr=args.stop,
+                                                        check_every=args.check_every,
+                                                        fallback_thr=fb_thr, cmax=cm)
+            else:
+                R = ragged.from_clouds(X, Y, regs)
+                u0 = ws_mod.predict_u0(net, R, X, Y, regs) if warm else None
+                e0.record()
+                res = ragged.sinkhorn_log_ragged(R, u0=u0, numItermax=args.max_iter,
+                                                 stopThr=args.stop, waves=args.waves,
+                                                 prefetch=args.prefetch,
+                                                 check_every=args.check_every,
+                                                 occupancy=args.occ, rotate=args.rotate,
+                                                 fallback_thr=fb_thr)
             e1.record()
             torch.cuda.synchronize()
             if k < args.warmup:
@@ -115,7 +130,12 @@ def run_warmstart(args, torch, dev):
             sweeps = nit + 2  # prologue + (niter + 1) fused sweeps
             if warm:
                 sweeps = sweeps + (st & 4 > 0) * 2  # a rejected warm start cost 2 sweeps
+            if args.cost == "points":
+                # over the points: the cost-maximum pass (cold) and a two-pass prologue
+                sweeps = sweeps + (0 if warm else 1) + 1
             alg_bytes += float(np.sum(sweeps * R.ns.astype(np.int64) * R.ms * 4))
+            elems += float(np.sum((nit + 1) * R.ns.astype(np.int64) * R.ms))
+            sweeps_tot += float(np.sum(sweeps * R.ns.astype(np.int64) * R.ms))
             iters.append(nit)
             conv.append((st & 3) == 0)
             fb.append((st & 4) > 0)
@@ -124,6 +144,7 @@ def run_warmstart(args, torch, dev):
         nprob = args.steps * B
         return {"problems_per_s": nprob / t_total, "ms_per_step": 1e3 * t_total / args.steps,
                 "kernel_ms": kern_ms / args.steps, "alg_bytes": alg_bytes / args.steps,
+                "sweep_elements": elems / args.steps, "pass_elements": sweeps_tot / args.steps,
                 "iters": {"mean": float(iters.mean()), "p50": float(np.percentile(iters, 50)),
                           "p90": float(np.percentile(iters, 90)), "max": int(iters.max())},
                 "converged_frac": float(conv.mean()),
@@ -140,6 +161,26 @@ def run_warmstart(args, torch, dev):
     gbs = cold["alg_bytes"] / (cold["kernel_ms"] * 1e-3) / 1e9
     traffic = 3.3667e8 * 1024 * 2.000 + 2.295e5 * 1024 * 1.000 \
         if (B == 4096 and args.stop == 1e-6 and args.max_iter == 2000) else None
+    if args.cost == "points":
+        # The point-cloud kernel streams no cost: per element of every pass it
+        # issues 4 MFMA products of 32x32x16 (f16 hi.lo, lo.hi, hi.hi and the
+        # bf16 potential slot), 128 flops, and in the fused sweeps one exp2.
+        secs = cold["kernel_ms"] * 1e-3
+        tf = 128.0 * cold["pass_elements"] / secs / 1e12
+        roof = {"bound": "mfma", "kernel": "batch_points_kernel (cold launch)",
+                "achieved": tf, "peak": MFMA_PEAK_TFLOPS["f16"], "unit": "TFLOP/s",
+                "frac": tf / MFMA_PEAK_TFLOPS["f16"], "traffic": None,
+                "flops_per_launch": 128.0 * cold["pass_elements"],
+                "avg_launch_ms": cold["kernel_ms"],
+                "exp2_per_s": cold["sweep_elements"] / secs,
+                "note": "executed MFMA flops of the cold launch; the points stay in LDS and "
+                        "registers, so HBM carries only points and potentials"}
+    else:
+        roof = {"bound": "hbm", "kernel": "batch_log_kernel (cold launch)",
+                "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                "alg_bytes_per_launch": cold["alg_bytes"],
+                "avg_launch_ms": cold["kernel_ms"]}
     out = {
         "value": head["problems_per_s"],
         "unit": "problems/s",
@@ -148,16 +189,12 @@ def run_warmstart(args, torch, dev):
         "config": {"workload": "warmstart-family n=m=1024 d=16 GMM(1-8) reg~logU[5e-3,5e-2]",
                    "problems_per_step": B, "stopThr": args.stop, "numItermax": args.max_iter,
                    "start": "learned" if warm else "cold (network weights absent)",
-                   "check_every": args.check_every,
+                   "check_every": args.check_every, "cost": args.cost,
                    "seed_family": 1_000_000},
         "cold": cold,
         "warm": warm,
         "speedup_warm_vs_cold": (warm["problems_per_s"] / cold["problems_per_s"]) if warm else None,
-        "roofline": {"bound": "hbm", "kernel": "batch_log_kernel (cold launch)",
-                     "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": cold["alg_bytes"],
-                     "avg_launch_ms": cold["kernel_ms"]},
+        "roofline": roof,
     }
     if not args.no_cpu:
         # a bounded sample: the first 8 problems of the evaluation set

@@ -171,6 +171,47 @@ int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int max_m, const 
                               int* niter, int* status,
                               double* err_last, double* err_first, int waves, void* stream);
 
+/* The same batch of sinkhorn_log problems given as point clouds: the squared
+ * Euclidean cost |x_r - y_c|^2 (ot.dist, ot/utils.py:270-307) is computed on
+ * the fly on MFMA and never stored; with `normalize` it is divided by its
+ * maximum (ot.dist(...) / M.max(), found on the device and returned in
+ * cmax).  Replaces, like potamd_sinkhorn_log_batch, a per-problem
+ * ot.bregman.sinkhorn_log call (ot/bregman/_sinkhorn.py:690-918) with the
+ * reference's stop rule, or the ot.batch solvers (ot/batch/_linear.py:
+ * 236-357) on uneven sizes.
+ * X: (sum n, d) float32 row-major, problem p's points from row xoff[p];
+ *    Y likewise with yoff.  a, u share X's offsets, b, v share Y's.
+ * n, m <= 1024, d <= 16.  regs: reg of each problem (on the normalised cost
+ *    when normalize != 0).  normalize: 0 the raw cost, 1 the cost divided by
+ *    its maximum (found here, written to cmax), 2 divided by the given cmax[p].
+ *    u: in u0 (natural log; 0 for the cold start), out log u; v: out log v.
+ * max_iter, check_every, stop_thr, fallback_thr, order, niter, status,
+ *    err_last, err_first: as for potamd_sinkhorn_log_batch; status 2 means
+ *    the problem was not solved (cost maximum 0, or reg so far below the
+ *    cost scale that the scaled coordinates leave the f16 range), with NaN
+ *    potentials.
+ * cmax: per-problem cost maximum (device double[B]): written with normalize
+ *    0 (1.0) or 1, read with 2; may be NULL unless normalize == 2.
+ */
+int potamd_sinkhorn_log_batch_points(int batch, int max_n, int max_m, int d, const float* X,
+                                     const float* Y, const int64_t* xoff, const int64_t* yoff,
+                                     const int* ns, const int* ms, const double* regs,
+                                     int normalize, const double* a, const double* b, double* u,
+                                     double* v, int max_iter, int check_every, double stop_thr,
+                                     double fallback_thr, const int* order, int* niter,
+                                     int* status, double* err_last, double* err_first,
+                                     double* cmax, void* stream);
+
+/* Per-problem maximum of |x_r - y_c|^2 (the M.max() of ot.dist(x, y) /
+ * M.max()), the same device search as potamd_sinkhorn_log_batch_points with
+ * normalize = 1 (MFMA pass, then the best pair of each lane evaluated in fp64
+ * from the points), without the solve.  cmax: device double[B]; a problem
+ * with n or m outside [1, 1024] gets NaN.
+ */
+int potamd_ragged_points_cmax(int batch, int max_n, int max_m, int d, const float* X,
+                              const float* Y, const int64_t* xoff, const int64_t* yoff,
+                              const int* ns, const int* ms, double* cmax, void* stream);
+
 /* ---- learned warm start (no counterpart in the reference) ------------- */
 
 /* bf16 MFMA GEMM with fused epilogue, the linear layers of the warm-start

@@ -42,6 +42,14 @@ SIGNATURES = {
         c_int, c_int, c_dbl, c_dbl, c_vp, c_int,  # max_iter, check_every, stop, fallback, order, vec
         c_vp, c_vp, c_vp, c_vp,              # niter, status, err_last, err_first
         c_int, c_vp]),                       # waves, stream
+    "potamd_sinkhorn_log_batch_points": (c_int, [
+        c_int, c_int, c_int, c_int, c_vp, c_vp,  # batch, max_n, max_m, d, X, Y
+        c_vp, c_vp, c_vp, c_vp, c_vp, c_int,     # xoff, yoff, ns, ms, regs, normalize
+        c_vp, c_vp, c_vp, c_vp,                  # a, b, u, v
+        c_int, c_int, c_dbl, c_dbl, c_vp,        # max_iter, check_every, stop, fallback, order
+        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),    # niter, status, err_last, err_first, cmax, stream
+    "potamd_ragged_points_cmax": (c_int, [
+        c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "potamd_sqeuclidean_ragged": (c_int, [
         c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
         c_vp, c_vp, c_vp, c_vp, c_vp]),

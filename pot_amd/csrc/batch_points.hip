@@ -1,0 +1,736 @@
+// Ragged batch of sinkhorn_log problems given as point clouds, with the
+// squared Euclidean cost computed on the fly (never stored).
+//
+// Each workgroup solves one problem (n, m <= 1024, d <= 16) with the
+// reference's sinkhorn_log loop (ot/bregman/_sinkhorn.py:880-906: v first,
+// then u, the column-marginal check every `check_every` iterations, strict
+// `err < stopThr`, niter = ii), exactly as batch_log_kernel does over a
+// materialised cost, on the cost M / max M (ot.dist then M / M.max(), the
+// reference's normalisation) or on M itself.
+//
+// Exponent of an element in base 2, with s = log2(e) / (reg * cmax) and the
+// points centred on the midpoint of the two centroids (M is unchanged):
+//     x_rc - shift_r = X_r . Y_c + (v_c - s|y_c|^2) - (shift_r + s|x_r|^2)
+// where X = sqrt(2 s) x and Y = sqrt(2 s) y.  The dot product runs on
+// 32x32x16 f16 MFMA with each coordinate split into f16 hi + lo (three
+// products hi.hi + hi.lo + lo.hi, the lo.lo term is below fp32 rounding).
+// The two potential terms are split into four bf16 parts each and added by
+// one more (bf16) MFMA whose K slots hold [P1..P4, 1, 1, 1, 1] against
+// [1, 1, 1, 1, -Q1..-Q4]; issued last, it cancels the large dot product
+// inside the accumulator, so the fp32 result is the small exponent itself.
+//
+// One sweep per iteration, as in the materialised kernels: the row sums use
+// the previous row log-sum-exp as shift (one exp2 per element), and the
+// column sums of P(u_new, v) come from the same registers weighted by
+// a_r / S_r.  Layout: the 8 waves own the column tiles (wave w owns tiles
+// w, w+8, w+16, w+24 of 32 columns; its C operands stay in registers for the
+// whole solve); all waves step through the rows 32 at a time, reading the
+// row operands from LDS.  A row sum is complete once the 8 waves' partials
+// meet in LDS (one barrier per block of 32 rows); the weighted column sums
+// accumulate per lane over all row blocks and are reduced across lanes once
+// per sweep.  A block whose row sums leave [2^-60, 2^60] is redone with its
+// rows re-centred on their maximum, as in the materialised kernels.
+//
+// cmax (normalised cost) is found on the device: a max pass over all pairs
+// on the same MFMA path picks each lane's largest pair, whose cost is then
+// evaluated exactly in fp64 from the points.
+#include <math.h>
+#include <string.h>
+
+#include <type_traits>
+
+#include "../../include/pot_amd.h"
+#include "host.h"
+#include "internal.h"
+
+namespace potamd {
+namespace {
+
+constexpr int kPW = 8;                       // wavefronts per problem
+constexpr int kPT = kPW * kWave;             // threads per problem
+constexpr int kPMax = 1024;                  // largest n and m
+constexpr int kPD = 16;                      // coordinates per point (zero padded)
+constexpr int kPBlocks = kPMax / 32;         // row blocks of 32
+constexpr int kPSlots = kPMax / (32 * kPW);  // column tiles per wave
+constexpr float kMaxOperand = 16384.f;       // largest |X|, |Y| accepted (f16 range)
+constexpr unsigned kOnes = 0x3F803F80u;      // two bf16 1.0
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+struct PointsArgs {
+  const float* X;        // packed points, problem p's rows at xoff[p] (row stride d)
+  const float* Y;
+  const int64_t* xoff;   // [B] also the offset of problem p in a and u
+  const int64_t* yoff;   // [B] also the offset in b and v
+  const int* ns;
+  const int* ms;
+  int d;
+  int normalize;         // 0 raw cost, 1 cost / max cost (max found here), 2 cost / cmax[p]
+  int max_only;          // 1: only find the cost maxima (into cmax)
+  const double* regs;    // [B]
+  const double* a;
+  const double* b;
+  double* u;             // in: u0 (natural log), out: log u
+  double* v;             // out: log v
+  const int* order;
+  int max_iter;
+  int check_every;
+  double stop_thr;
+  double fallback_thr;
+  int* niter;
+  int* status;
+  double* err_last;
+  double* err_first;
+  double* cmax;          // [B] the cost maximum used (out; in when normalize == 2); may be null
+};
+
+__device__ __forceinline__ unsigned bf16_bits(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// x = p[0] + p[1] + p[2] + p[3] to 2^-32 relative, each part a bf16; an
+// infinite x gives (x, 0, 0, 0).
+__device__ __forceinline__ void split4_bf16(double x, unsigned p[4]) {
+  if (!__builtin_isfinite(x)) {
+    p[0] = bf16_bits((float)x);
+    p[1] = p[2] = p[3] = 0u;
+    return;
+  }
+  double r = x;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    p[i] = bf16_bits((float)r);
+    r -= (double)__uint_as_float(p[i] << 16);
+  }
+}
+
+// K slots of the potential MFMA.  Column side (A operand, lanes 0-31):
+// [P1, P2, P3, P4, 1, 1, 1, 1]; row side (B operand): [1, 1, 1, 1, -Q1..-Q4].
+__device__ __forceinline__ u32x4 kslot_col(double P) {
+  unsigned p[4];
+  split4_bf16(P, p);
+  u32x4 k;
+  k[0] = p[0] | (p[1] << 16);
+  k[1] = p[2] | (p[3] << 16);
+  k[2] = kOnes;
+  k[3] = kOnes;
+  return k;
+}
+__device__ __forceinline__ u32x4 kslot_row(double Q) {
+  unsigned q[4];
+  split4_bf16(-Q, q);
+  u32x4 k;
+  k[0] = kOnes;
+  k[1] = kOnes;
+  k[2] = q[0] | (q[1] << 16);
+  k[3] = q[2] | (q[3] << 16);
+  return k;
+}
+
+// Half a point (coordinates 8*hh .. 8*hh+7) as f16 hi and lo operands of
+// (p - cen) * sc; nrm accumulates the squared norm of the represented
+// (hi + lo) coordinates, mx the largest |coordinate|.
+__device__ __forceinline__ void point_ops(const float* p, bool valid, int d, const double* cen,
+                                          double sc, int hh, u32x4& hi, u32x4& lo, double& nrm,
+                                          float& mx) {
+  unsigned hb[8], lb[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int kk = 8 * hh + k;
+    double x = 0.0;
+    if (valid && kk < d) x = ((double)p[kk] - cen[kk]) * sc;
+    const _Float16 xh = (_Float16)(float)x;
+    const double xhd = (double)(float)xh;
+    const _Float16 xl = (_Float16)(float)(x - xhd);
+    const double xr = xhd + (double)(float)xl;
+    nrm = fma(xr, xr, nrm);
+    mx = fmaxf(mx, fabsf((float)x));
+    hb[k] = __builtin_bit_cast(unsigned short, xh);
+    lb[k] = __builtin_bit_cast(unsigned short, xl);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    hi[k] = hb[2 * k] | (hb[2 * k + 1] << 16);
+    lo[k] = lb[2 * k] | (lb[2 * k + 1] << 16);
+  }
+}
+
+__device__ __forceinline__ f32x16 mfma_h(u32x4 a, u32x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h16x8, a),
+                                                __builtin_bit_cast(h16x8, b), c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma_b(u32x4 a, u32x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// Reduction over the 32 lanes of each wave half (the lanes that share a
+// column of an MFMA tile): four row-local DPP steps, then row 0 into row 1
+// and row 2 into row 3 (row_bcast:15).  Lanes 16-31 and 48-63 end with the
+// totals of their halves.
+template <bool MAX>
+__device__ __forceinline__ float half_reduce(float x) {
+  auto op = [](float p, float q) { return MAX ? fmaxf(p, q) : p + q; };
+  x = op(x, dpp_f<0xB1>(x));
+  x = op(x, dpp_f<0x4E>(x));
+  x = op(x, dpp_f<0x141>(x));
+  x = op(x, dpp_f<0x140>(x));
+  const float id = MAX ? -__builtin_inff() : 0.f;
+  const float y = __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(id), __float_as_int(x), 0x142, 0xA, 0xF, false));
+  return op(x, y);
+}
+
+struct PointsLds {
+  u32x4 rop[kPBlocks][2][kWave];  // row operands: [block][hi, lo][lane]
+  u32x4 rks[kPMax];               // row K slots
+  double c2s[kPMax];              // row shifts log2 a - u2 (base 2)
+  double xn[kPMax];               // s |x_r|^2
+  double vsh[kPMax];              // column potentials (base 2)
+  double yn[kPMax];               // s |y_c|^2
+  float af[kPMax];                // row weights
+  float sh[kPMax];                // shifted row sums (prologue: column maxima)
+  float colx[kPMax];              // column sums
+  float rowpart[2][kPW][32];
+  float rowmax[kPW][32];
+  double cen[kPD];
+  double red[kPW];
+  float redf[kPW];
+  int flag;
+};
+
+// Everything after the centring, for problems whose column tiles take NS
+// slots per wave (NS = ceil(ceil(m / 32) / 8)); slots past m hold padding
+// columns whose potential is -inf, so they add nothing.
+template <int NS>
+__device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, int p, int n,
+                                             int m, double ext) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int h = lane >> 5, l32 = lane & 31;
+  const int d = A.d;
+  const int64_t xo = A.xoff[p], yo = A.yoff[p];
+  const float* Xp = A.X + xo * d;
+  const float* Yp = A.Y + yo * d;
+  const double* ap = A.a + xo;
+  const double* bp = A.b + yo;
+  double* up = A.u + xo;
+  double* vp = A.v + yo;
+  const int nblk = (n + 31) / 32;
+  // Tile j of this wave covers columns 32 * (wave + 8 j) + [0, 32).
+  auto col_of = [&](int j) { return 32 * (wave + kPW * j) + l32; };
+  // Column of value i of tile j in the MFMA output layout.
+  auto out_col = [&](int j, int i) {
+    return 32 * (wave + kPW * j) + 8 * (i / 4) + 4 * h + (i % 4);
+  };
+
+  // Operand packing: rows into rop / xn, this wave's columns into registers / yn.
+  u32x4 ahi[NS], alo[NS], ak[NS];
+  auto pack = [&](double sc) -> float {
+    float mx = 0.f;
+    for (int q = tid; q < nblk * kWave; q += kPT) {
+      const int blk = q >> 6, ln = q & 63, r = blk * 32 + (ln & 31), hh = ln >> 5;
+      u32x4 hi4, lo4;
+      double nh = 0.0;
+      point_ops(Xp + (int64_t)(r < n ? r : 0) * d, r < n, d, L.cen, sc, hh, hi4, lo4, nh, mx);
+      L.rop[blk][0][ln] = hi4;
+      L.rop[blk][1][ln] = lo4;
+      nh += __shfl_xor(nh, 32, kWave);
+      if (hh == 0) L.xn[r] = r < n ? 0.5 * nh : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int c = col_of(j);
+      u32x4 hi4, lo4;
+      double nh = 0.0;
+      point_ops(Yp + (int64_t)(c < m ? c : 0) * d, c < m, d, L.cen, sc, h, hi4, lo4, nh, mx);
+      ahi[j] = hi4;
+      alo[j] = lo4;
+      nh += __shfl_xor(nh, 32, kWave);
+      if (h == 0) L.yn[c] = c < m ? 0.5 * nh : 0.0;
+    }
+    return mx;
+  };
+  const u32x4 zero4 = {0u, 0u, 0u, 0u};
+  auto set_col_kslots = [&](auto P_of) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) ak[j] = h == 0 ? kslot_col(P_of(col_of(j))) : zero4;
+  };
+  // D[c][r] tiles of this wave's columns against row block blk; the small
+  // products first, the potential last.
+  f32x16 e[NS];
+  auto tiles = [&](int blk, u32x4 bk) {
+    const u32x4 bh = L.rop[blk][0][lane];
+    const u32x4 bl = L.rop[blk][1][lane];
+    const f32x16 z = {};
+#pragma unroll
+    for (int j = 0; j < NS; ++j) e[j] = mfma_h(alo[j], bh, z);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) e[j] = mfma_h(ahi[j], bl, e[j]);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) e[j] = mfma_h(ahi[j], bh, e[j]);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) e[j] = mfma_b(ak[j], bk, e[j]);
+  };
+  // e <- exp2(e); returns the lane's row sum over this wave's columns (both
+  // halves of the wave hold the same row), one partial per tile.
+  auto exp_rowsum = [&]() {
+    float part[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      part[j] = 0.f;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        e[j][i] = exp2_fast(e[j][i]);
+        part[j] += e[j][i];
+      }
+    }
+    float sl = part[0];
+#pragma unroll
+    for (int j = 1; j < NS; ++j) sl += part[j];
+    return sl + __shfl_xor(sl, 32, kWave);
+  };
+  auto row_kslot = [&](int blk) {
+    const u32x4 k = L.rks[blk * 32 + l32];
+    return h == 0 ? k : zero4;
+  };
+  // Per-column reduction of acc over the rows (sum or max) into dst.
+  f32x16 acc[NS];
+  auto columns_out = [&](auto max_tag, float* dst) {
+    constexpr bool MAX = decltype(max_tag)::value;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      float w = 0.f;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const float t = half_reduce<MAX>(acc[j][i]);
+        w = (l32 == 16 + i) ? t : w;
+      }
+      if (l32 >= 16) {
+        const int i = l32 - 16;
+        dst[32 * (wave + kPW * j) + 8 * (i / 4) + 4 * h + (i % 4)] = w;
+      }
+    }
+  };
+
+  // ---- cost maximum ----
+  double cmax = A.normalize == 2 ? A.cmax[p] : 1.0;
+  if (A.normalize == 1) {
+    const double sc0 = 1.4142135623730951 * (ext > 0.0 ? ldexp(1.0, -ilogb(ext) - 1) : 1.0);
+    pack(sc0);
+    __syncthreads();
+    // Padding is +inf so that it never wins the minimum of -cost.
+    set_col_kslots([&](int c) { return c < m ? -L.yn[c] : __builtin_inf(); });
+    for (int r = tid; r < nblk * 32; r += kPT)
+      L.rks[r] = kslot_row(r < n ? L.xn[r] : -__builtin_inf());
+    __syncthreads();
+    float best = __builtin_inff();
+    int code = -1;
+    for (int blk = 0; blk < nblk; ++blk) {
+      tiles(blk, row_kslot(blk));
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const bool lt = e[j][i] < best;
+          best = lt ? e[j][i] : best;
+          code = lt ? (blk << 8) | (j << 4) | i : code;
+        }
+      }
+    }
+    double mexact = 0.0;
+    if (code >= 0) {
+      const int blk = code >> 8, j = (code >> 4) & 15, i = code & 15;
+      const int r = blk * 32 + l32, c = out_col(j, i);
+      if (r < n && c < m) {
+        for (int k = 0; k < d; ++k) {
+          const double t = (double)Xp[(int64_t)r * d + k] - (double)Yp[(int64_t)c * d + k];
+          mexact = fma(t, t, mexact);
+        }
+      }
+    }
+    mexact = wave_max(mexact);
+    if (lane == 0) L.red[wave] = mexact;
+    __syncthreads();
+    cmax = 0.0;
+    for (int w = 0; w < kPW; ++w) cmax = fmax(cmax, L.red[w]);
+    __syncthreads();
+    if (A.max_only) {
+      if (tid == 0) A.cmax[p] = cmax;
+      return;
+    }
+  }
+  const double reg = A.regs[p] * cmax;
+  const double s = kLog2e / reg;
+  {
+    const float wm = wave_max(pack(sqrt(2.0 * s)));
+    if (lane == 0) L.redf[wave] = wm;
+  }
+  __syncthreads();
+  float allmax = 0.f;
+  for (int w = 0; w < kPW; ++w) allmax = fmaxf(allmax, L.redf[w]);
+  if (!(allmax <= kMaxOperand) || !(cmax > 0.0) || !(reg > 0.0)) {
+    // Outside the operand range of the f16 MFMA path (reg far below the
+    // cost scale) or a degenerate cost: report instead of solving.
+    for (int i = tid; i < n; i += kPT) up[i] = __builtin_nan("");
+    for (int j = tid; j < m; j += kPT) vp[j] = __builtin_nan("");
+    if (tid == 0) {
+      A.niter[p] = 0;
+      A.status[p] = kNumerical;
+      if (A.cmax && A.normalize != 2) A.cmax[p] = cmax;
+    }
+    return;
+  }
+
+  int status = kMaxIter, niter = A.max_iter - 1, restarted = 0;
+  double err_last = -1.0, err_first = -1.0;
+  const bool warm = A.fallback_thr > 0.0;
+  bool cold = false;
+
+restart:
+  // ---- prologue: v_1 = log2 b - LSE2_r(x_rc + u2_r), a max pass then a sum pass ----
+  for (int r = tid; r < nblk * 32; r += kPT) {
+    const double u2 = (r < n && !cold) ? up[r] * kLog2e : 0.0;
+    L.rks[r] = kslot_row(r < n ? L.xn[r] - u2 : __builtin_inf());
+  }
+  set_col_kslots([&](int c) { return c < m ? -L.yn[c] : -__builtin_inf(); });
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NS; ++j)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[j][i] = -__builtin_inff();
+  for (int blk = 0; blk < nblk; ++blk) {
+    tiles(blk, row_kslot(blk));
+#pragma unroll
+    for (int j = 0; j < NS; ++j) acc[j] = __builtin_elementwise_max(acc[j], e[j]);
+  }
+  columns_out(std::integral_constant<bool, true>(), L.sh);
+  __syncthreads();
+  set_col_kslots([&](int c) {
+    const float g = L.sh[c < m ? c : 0];
+    return (c < m && __builtin_isfinite(g)) ? -L.yn[c] - (double)g : -__builtin_inf();
+  });
+#pragma unroll
+  for (int j = 0; j < NS; ++j) acc[j] = f32x16{};
+  for (int blk = 0; blk < nblk; ++blk) {
+    tiles(blk, row_kslot(blk));
+#pragma unroll
+    for (int j = 0; j < NS; ++j)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[j][i] += exp2_fast(e[j][i]);
+  }
+  columns_out(std::integral_constant<bool, false>(), L.colx);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int c = col_of(j);
+    if (h == 0 && c < m) {
+      const double bj = bp[c];
+      const double g = (double)L.sh[c];
+      L.vsh[c] = (bj > 0.0 && __builtin_isfinite(g)) ? log2(bj) - (g + log2((double)L.colx[c]))
+                                                     : -__builtin_inf();
+    }
+  }
+  set_col_kslots([&](int c) { return c < m ? L.vsh[c] - L.yn[c] : -__builtin_inf(); });
+  for (int r = tid; r < nblk * 32; r += kPT) {
+    double c2 = __builtin_inf();
+    if (r < n && L.af[r] > 0.f) {
+      const double u2 = cold ? 0.0 : up[r] * kLog2e;
+      c2 = log2(ap[r]) - u2;
+    }
+    L.c2s[r] = c2;
+    L.rks[r] = kslot_row(r < n ? c2 + L.xn[r] : __builtin_inf());
+  }
+  __syncthreads();
+
+  for (int ii = 0; ii < A.max_iter; ++ii) {
+    // ---- fused sweep: row sums (u update) and column sums of P(u_new, v) ----
+#pragma unroll
+    for (int j = 0; j < NS; ++j) acc[j] = f32x16{};
+    for (int blk = 0; blk < nblk; ++blk) {
+      u32x4 bk = row_kslot(blk);
+      tiles(blk, bk);
+      float sl = exp_rowsum();
+      const int buf = blk & 1;
+      if (h == 0) L.rowpart[buf][wave][l32] = sl;
+      __syncthreads();
+      float S = 0.f;
+#pragma unroll
+      for (int w = 0; w < kPW; ++w) S += L.rowpart[buf][w][l32];
+      const int r = blk * 32 + l32;
+      const float ar = L.af[r];
+      const bool bad = ar > 0.f && !in_safe_range((double)S);
+      if (__ballot(bad)) {
+        // Re-centre the rows that left the range on their maximum exponent.
+        tiles(blk, bk);
+        float mx = -__builtin_inff();
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+#pragma unroll
+          for (int i = 0; i < 16; ++i) mx = fmaxf(mx, e[j][i]);
+        mx = fmaxf(mx, __shfl_xor(mx, 32, kWave));
+        if (h == 0) L.rowmax[wave][l32] = mx;
+        __syncthreads();
+        float G = -__builtin_inff();
+#pragma unroll
+        for (int w = 0; w < kPW; ++w) G = fmaxf(G, L.rowmax[w][l32]);
+        double c2 = L.c2s[r];
+        if (bad && __builtin_isfinite(G)) {
+          c2 += (double)G;
+          const u32x4 k = kslot_row(c2 + L.xn[r]);
+          bk = h == 0 ? k : zero4;
+        }
+        tiles(blk, bk);
+        sl = exp_rowsum();
+        if (h == 0) L.rowpart[buf][wave][l32] = sl;
+        __syncthreads();
+        S = 0.f;
+#pragma unroll
+        for (int w = 0; w < kPW; ++w) S += L.rowpart[buf][w][l32];
+        if (wave == 0 && h == 0 && bad) {
+          L.c2s[r] = c2;  // the shift actually used
+          L.rks[r] = bk;
+        }
+      }
+      const float wr = ar > 0.f ? ar * __builtin_amdgcn_rcpf(S) : 0.f;
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[j][i] = __builtin_fmaf(e[j][i], wr, acc[j][i]);
+      if (wave == 0 && h == 0) L.sh[r] = S;
+    }
+    columns_out(std::integral_constant<bool, false>(), L.colx);
+    __syncthreads();
+
+    // ---- row potentials, marginal error, stop decision ----
+    for (int r = tid; r < n; r += kPT) {
+      if (L.af[r] > 0.f) {
+        const double c = L.c2s[r] + log2((double)L.sh[r]);
+        L.c2s[r] = c;
+        L.rks[r] = kslot_row(c + L.xn[r]);
+      }
+    }
+    double e2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int c = col_of(j);
+      if (h == 0 && c < m) {
+        const double dd = (double)L.colx[c] - bp[c];
+        e2 += dd * dd;
+      }
+    }
+    e2 = wave_sum(e2);
+    if (lane == 0) L.red[wave] = e2;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kPW; ++w) t += L.red[w];
+      const double err = sqrt(t);
+      int stop = 0;
+      if (ii % A.check_every == 0) {
+        err_last = err;
+        if (ii == 0) {
+          err_first = err;
+          if (warm && !restarted && err > A.fallback_thr) stop = 2;
+        }
+        if (stop == 0 && err < A.stop_thr) stop = 1;
+      }
+      L.flag = stop;
+    }
+    __syncthreads();
+    const int fl = L.flag;
+    if (fl == 2) {
+      // Warm start rejected: restart from the cold start u0 = 0.
+      restarted = 1;
+      cold = true;
+      __syncthreads();
+      goto restart;
+    }
+    if (fl == 1) {
+      status = kConverged;
+      niter = ii;
+      break;
+    }
+    if (ii == A.max_iter - 1) break;
+    // ---- column update v_c += log2 b_c - log2 colsum_c (own columns) ----
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int c = col_of(j);
+      if (h == 0 && c < m) {
+        const double bj = bp[c];
+        const double vn =
+            bj > 0.0 ? L.vsh[c] + log2(bj) - log2((double)L.colx[c]) : -__builtin_inf();
+        L.vsh[c] = vn;
+        ak[j] = kslot_col(vn - L.yn[c]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- outputs (natural-log potentials) ----
+  __syncthreads();
+  for (int i = tid; i < n; i += kPT)
+    up[i] = L.af[i] > 0.f ? (log2(ap[i]) - L.c2s[i]) * kLn2 : -__builtin_inf();
+  for (int j = tid; j < m; j += kPT) vp[j] = L.vsh[j] * kLn2;
+  if (tid == 0) {
+    A.niter[p] = niter;
+    A.status[p] = status | (restarted ? 4 : 0);
+    if (A.err_last) A.err_last[p] = err_last;
+    if (A.err_first) A.err_first[p] = err_first;
+    if (A.cmax && A.normalize != 2) A.cmax[p] = cmax;
+  }
+}
+
+__global__ __launch_bounds__(kPT) void batch_points_kernel(PointsArgs A) {
+  __shared__ PointsLds L;
+  const int p = A.order ? A.order[blockIdx.x] : blockIdx.x;
+  const int tid = threadIdx.x;
+  const int n = A.ns[p], m = A.ms[p], d = A.d;
+  if (n < 1 || n > kPMax || m < 1 || m > kPMax) {
+    if (tid == 0) {
+      if (A.max_only) {
+        A.cmax[p] = __builtin_nan("");
+      } else {
+        A.niter[p] = 0;
+        A.status[p] = kNumerical;
+      }
+    }
+    return;
+  }
+  const float* Xp = A.X + A.xoff[p] * d;
+  const float* Yp = A.Y + A.yoff[p] * d;
+  if (!A.max_only) {
+    const double* ap = A.a + A.xoff[p];
+    for (int i = tid; i < kPMax; i += kPT) L.af[i] = i < n ? (float)ap[i] : 0.f;
+  }
+
+  // ---- centre (midpoint of the centroids) and coordinate extent ----
+  double ext = 0.0;  // largest |coordinate - centre| over both clouds
+  {
+    double* scr = reinterpret_cast<double*>(&L.rop[0][0][0]);
+    const int k = tid & 15, g = tid >> 4;
+    double sx = 0.0, sy = 0.0;
+    float lo = __builtin_inff(), hi = -__builtin_inff();
+    if (k < d) {
+      for (int i = g; i < n; i += 32) {
+        const float x = Xp[(int64_t)i * d + k];
+        sx += x;
+        lo = fminf(lo, x);
+        hi = fmaxf(hi, x);
+      }
+      for (int i = g; i < m; i += 32) {
+        const float y = Yp[(int64_t)i * d + k];
+        sy += y;
+        lo = fminf(lo, y);
+        hi = fmaxf(hi, y);
+      }
+    }
+    scr[tid] = sx;
+    scr[kPT + tid] = sy;
+    scr[2 * kPT + tid] = lo;
+    scr[3 * kPT + tid] = hi;
+    __syncthreads();
+    if (tid < kPD) {
+      double cx = 0.0, cy = 0.0, l = __builtin_inf(), u = -__builtin_inf();
+      for (int gg = 0; gg < 32; ++gg) {
+        cx += scr[gg * 16 + tid];
+        cy += scr[kPT + gg * 16 + tid];
+        l = fmin(l, scr[2 * kPT + gg * 16 + tid]);
+        u = fmax(u, scr[3 * kPT + gg * 16 + tid]);
+      }
+      const double c = tid < d ? 0.5 * (cx / n + cy / m) : 0.0;
+      L.cen[tid] = c;
+      scr[4 * kPT + tid] = tid < d ? fmax(u - c, c - l) : 0.0;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < kPD; ++kk) ext = fmax(ext, scr[4 * kPT + kk]);
+    __syncthreads();  // the scratch is the row-operand area, packed next
+  }
+  const int ns = ((m + 31) / 32 + kPW - 1) / kPW;
+  if (ns <= 1)
+    solve_points<1>(A, L, p, n, m, ext);
+  else if (ns == 2)
+    solve_points<2>(A, L, p, n, m, ext);
+  else if (ns == 3)
+    solve_points<3>(A, L, p, n, m, ext);
+  else
+    solve_points<4>(A, L, p, n, m, ext);
+}
+
+}  // namespace
+}  // namespace potamd
+
+using namespace potamd;
+
+extern "C" int potamd_sinkhorn_log_batch_points(
+    int batch, int max_n, int max_m, int d, const float* X, const float* Y, const int64_t* xoff,
+    const int64_t* yoff, const int* ns, const int* ms, const double* regs, int normalize,
+    const double* a, const double* b, double* u, double* v, int max_iter, int check_every,
+    double stop_thr, double fallback_thr, const int* order, int* niter, int* status,
+    double* err_last, double* err_first, double* cmax, void* stream) {
+  if (batch < 1 || max_n < 1 || max_m < 1 || d < 1 || !X || !Y || !xoff || !yoff || !ns ||
+      !ms || !regs || !a || !b || !u || !v || max_iter < 1 || !niter || !status ||
+      check_every < 0 || normalize < 0 || normalize > 2 || (normalize == 2 && !cmax))
+    return POTAMD_EINVAL;
+  if (max_n > kPMax || max_m > kPMax || d > kPD) return POTAMD_EUNSUPPORTED;
+  PointsArgs pa;
+  memset(&pa, 0, sizeof(pa));
+  pa.X = X;
+  pa.Y = Y;
+  pa.xoff = xoff;
+  pa.yoff = yoff;
+  pa.ns = ns;
+  pa.ms = ms;
+  pa.d = d;
+  pa.normalize = normalize;
+  pa.regs = regs;
+  pa.a = a;
+  pa.b = b;
+  pa.u = u;
+  pa.v = v;
+  pa.order = order;
+  pa.max_iter = max_iter;
+  pa.check_every = check_every > 0 ? check_every : 10;
+  pa.stop_thr = stop_thr;
+  pa.fallback_thr = fallback_thr;
+  pa.niter = niter;
+  pa.status = status;
+  pa.err_last = err_last;
+  pa.err_first = err_first;
+  pa.cmax = cmax;
+  hipLaunchKernelGGL(batch_points_kernel, dim3(batch), dim3(kPT), 0,
+                     static_cast<hipStream_t>(stream), pa);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+extern "C" int potamd_ragged_points_cmax(int batch, int max_n, int max_m, int d, const float* X,
+                                         const float* Y, const int64_t* xoff, const int64_t* yoff,
+                                         const int* ns, const int* ms, double* cmax,
+                                         void* stream) {
+  if (batch < 1 || max_n < 1 || max_m < 1 || d < 1 || !X || !Y || !xoff || !yoff || !ns ||
+      !ms || !cmax)
+    return POTAMD_EINVAL;
+  if (max_n > kPMax || max_m > kPMax || d > kPD) return POTAMD_EUNSUPPORTED;
+  PointsArgs pa;
+  memset(&pa, 0, sizeof(pa));
+  pa.X = X;
+  pa.Y = Y;
+  pa.xoff = xoff;
+  pa.yoff = yoff;
+  pa.ns = ns;
+  pa.ms = ms;
+  pa.d = d;
+  pa.normalize = 1;
+  pa.max_only = 1;
+  pa.cmax = cmax;
+  hipLaunchKernelGGL(batch_points_kernel, dim3(batch), dim3(kPT), 0,
+                     static_cast<hipStream_t>(stream), pa);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}

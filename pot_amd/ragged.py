@@ -81,8 +81,10 @@ def _layout(ns, ms, device):
 
 def _weights(ws, counts, device):
     if ws is None:
-        return torch.cat([torch.full((int(c),), 1.0 / int(c), dtype=torch.float64)
-                          for c in counts]).to(device)
+        counts = np.asarray(counts, dtype=np.int64)
+        inv = torch.from_numpy(1.0 / counts.astype(np.float64)).to(device)
+        return torch.repeat_interleave(inv, torch.from_numpy(counts).to(device),
+                                       output_size=int(counts.sum()))
     return torch.cat([torch.as_tensor(np.asarray(w), dtype=torch.float64).reshape(-1)
                       for w in ws]).to(device)
 
@@ -163,6 +165,7 @@ class RaggedResult:
     status: torch.Tensor     # (B,) int32: 0 converged, 1 max-iter
     err_first: torch.Tensor  # (B,) float64 marginal error at ii = 0
     err_last: torch.Tensor   # (B,) float64 last checked marginal error
+    cmax: torch.Tensor = None  # (B,) float64 cost maxima (point-cloud solver)
 
     def split(self, R, p):
         ao, bo = int(R.aoff[p].item()), int(R.boff[p].item())
@@ -216,21 +219,168 @@ def sinkhorn_log_ragged(R, u0=None, numItermax=1000, stopThr=1e-9, order=None,
                         err_last=e_last)
 
 
+MAX_D_POINTS = 16
+
+
+@dataclass
+class RaggedPoints:
+    """Packed point clouds of B problems for the on-the-fly cost solver."""
+    device: torch.device
+    ns: np.ndarray          # (B,) int
+    ms: np.ndarray
+    d: int
+    X: torch.Tensor         # (sum n, d) float32
+    Y: torch.Tensor         # (sum m, d) float32
+    aoff: torch.Tensor      # (B,) int64 row offsets (points, a and u)
+    boff: torch.Tensor      # (B,) int64 column offsets (points, b and v)
+    ns_d: torch.Tensor      # (B,) int32
+    ms_d: torch.Tensor
+    a: torch.Tensor         # (sum n,) float64
+    b: torch.Tensor         # (sum m,) float64
+    regs: torch.Tensor      # (B,) float64 reg (on the normalised cost when normalize)
+    normalize: bool
+    regs_host: np.ndarray = None  # (B,) float64 host copy of regs (launch order)
+    cmax: torch.Tensor = None  # (B,) float64 cost maxima once found (points_cmax)
+
+    @property
+    def batch(self):
+        return len(self.ns)
+
+
+def points_cmax(P):
+    """Per-problem maximum of |x - y|^2 (float64, on the device), found on the
+    GPU without solving; also stored in ``P.cmax``."""
+    lib = _lib.load()
+    cmax = torch.empty(P.batch, dtype=torch.float64, device=P.device)
+    rc = lib.potamd_ragged_points_cmax(
+        P.batch, int(P.ns.max()), int(P.ms.max()), P.d, _ptr(P.X), _ptr(P.Y), _ptr(P.aoff),
+        _ptr(P.boff), _ptr(P.ns_d), _ptr(P.ms_d), _ptr(cmax),
+        ctypes.c_void_p(_device.stream_ptr(P.device)))
+    _lib.check(rc, "potamd_ragged_points_cmax")
+    P.cmax = cmax
+    return cmax
+
+
+def points_from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None):
+    """Pack point-cloud pairs for ``sinkhorn_log_ragged_points``.
+
+    xs[p]: (n_p, d), ys[p]: (m_p, d) arrays or tensors (or (B, n, d) tensors
+    for equal sizes), n_p, m_p <= 1024, d <= 16; regs: (B,).  With
+    ``normalize`` each problem's cost is |x - y|^2 / max |x - y|^2.
+    """
+    device = device or _device.require_gpu()
+    if isinstance(xs, torch.Tensor) and isinstance(ys, torch.Tensor):
+        ns = [int(xs.shape[1])] * int(xs.shape[0])
+        ms = [int(ys.shape[1])] * int(ys.shape[0])
+    else:
+        ns = [len(x) for x in xs]
+        ms = [len(y) for y in ys]
+    d = int(xs[0].shape[-1])
+    if d > MAX_D_POINTS:
+        raise ValueError(f"the point-cloud solver supports d <= {MAX_D_POINTS}")
+    ns = np.asarray(ns, dtype=np.int64)
+    ms = np.asarray(ms, dtype=np.int64)
+    if ns.min() < 1 or ms.min() < 1:
+        raise ValueError("every problem needs at least one point on each side")
+    if ms.max() > MAX_M or ns.max() > MAX_N:
+        raise ValueError(f"ragged solver supports n <= {MAX_N}, m <= {MAX_M}")
+    aoff = np.concatenate([[0], np.cumsum(ns)[:-1]])
+    boff = np.concatenate([[0], np.cumsum(ms)[:-1]])
+    t = lambda x, dt: torch.tensor(x, dtype=dt, device=device)
+    regs_h = np.asarray(regs, dtype=np.float64).reshape(-1)
+    regs_t = torch.from_numpy(regs_h.copy()).to(device)
+    return RaggedPoints(device=device, ns=ns, ms=ms, d=d, X=_pack_points(xs, device),
+                        Y=_pack_points(ys, device), aoff=t(aoff, torch.int64),
+                        boff=t(boff, torch.int64), ns_d=t(ns, torch.int32),
+                        ms_d=t(ms, torch.int32), a=_weights(a, ns, device),
+                        b=_weights(b, ms, device), regs=regs_t, normalize=bool(normalize),
+                        regs_host=regs_h)
+
+
+def _check_order(order, B):
+    """A launch order must be a permutation of range(B) (it indexes problems)."""
+    o = order.cpu().numpy() if isinstance(order, torch.Tensor) else np.asarray(order)
+    o = o.astype(np.int64).reshape(-1)
+    if o.shape != (B,) or not np.array_equal(np.sort(o), np.arange(B)):
+        raise ValueError("order must be a permutation of the problem indices")
+    return o.astype(np.int32)
+
+
+def _points_order(ns, ms, regs):
+    work = ns.astype(np.float64) * ms / np.asarray(regs, dtype=np.float64)
+    return np.argsort(-work, kind="stable").astype(np.int32)
+
+
+def _launch_points(lib, P, X, Y, regs, u, v, niter, status, e_last, e_first, cmax, order,
+                   numItermax, check_every, stopThr, fallback_thr, stream, cmax_given=False):
+    mode = (2 if cmax_given else 1) if P.normalize else 0
+    rc = lib.potamd_sinkhorn_log_batch_points(
+        P.batch, int(P.ns.max()), int(P.ms.max()), P.d, _ptr(X), _ptr(Y), _ptr(P.aoff),
+        _ptr(P.boff), _ptr(P.ns_d), _ptr(P.ms_d), _ptr(regs), mode, _ptr(P.a),
+        _ptr(P.b), _ptr(u), _ptr(v), int(numItermax), int(check_every), float(stopThr),
+        float(fallback_thr), _ptr(order), _ptr(niter), _ptr(status), _ptr(e_last),
+        _ptr(e_first), _ptr(cmax), stream)
+    _lib.check(rc, "potamd_sinkhorn_log_batch_points")
+
+
+def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=None,
+                               fallback_thr=0.0, check_every=10, cmax=None):
+    """``sinkhorn_log_ragged`` on point clouds, the cost computed on the fly.
+
+    P: a RaggedPoints.  Same loop, stop rule, warm start and fallback as
+    ``sinkhorn_log_ragged``; the cost of problem p is |x - y|^2 (divided by
+    its maximum when P.normalize), evaluated on MFMA from the points in every
+    sweep and never stored.  ``cmax``: the maxima from ``points_cmax`` (the
+    solve then skips its own search).  Returns a RaggedResult whose ``cmax``
+    holds the cost maxima; asynchronous on the current stream.
+    """
+    lib = _lib.load()
+    dev = P.device
+    B = P.batch
+    u = torch.zeros(int(P.ns.sum()), dtype=torch.float64, device=dev) if u0 is None \
+        else u0.to(device=dev, dtype=torch.float64).contiguous().clone()
+    v = torch.empty(int(P.ms.sum()), dtype=torch.float64, device=dev)
+    niter = torch.empty(B, dtype=torch.int32, device=dev)
+    status = torch.empty(B, dtype=torch.int32, device=dev)
+    e_first = torch.empty(B, dtype=torch.float64, device=dev)
+    e_last = torch.empty(B, dtype=torch.float64, device=dev)
+    given = cmax is not None and P.normalize
+    cmax = cmax.to(device=dev, dtype=torch.float64).contiguous() if given \
+        else torch.empty(B, dtype=torch.float64, device=dev)
+    if order is None:
+        regs_h = P.regs_host if P.regs_host is not None else P.regs.cpu().numpy()
+        order = torch.from_numpy(_points_order(P.ns, P.ms, regs_h)).to(dev)
+    else:
+        order = torch.as_tensor(_check_order(order, B), device=dev)
+    _launch_points(lib, P, P.X, P.Y, P.regs, u, v, niter, status, e_last, e_first, cmax, order,
+                   numItermax, check_every, stopThr, fallback_thr,
+                   ctypes.c_void_p(_device.stream_ptr(dev)), cmax_given=given)
+    return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
+                        err_last=e_last, cmax=cmax)
+
+
 class RaggedGraph:
     """Cost construction and solve of a fixed ragged layout, captured once as a
     hipGraph (through ``torch.cuda.CUDAGraph``) and replayed per batch.
 
     The layout (sizes, d, iteration settings) is fixed at construction; each
     ``solve`` copies new points, regs and (optionally) initial potentials into
-    the graph's static buffers, sets the launch order, and replays the two
-    kernels (ragged squared distances with per-problem maxima, then the
-    batched sinkhorn_log).  The captured work contains no memset nodes and no
+    the graph's static buffers, sets the launch order, and replays the
+    kernels: with ``cost="dense"`` the ragged squared distances with
+    per-problem maxima, then the batched sinkhorn_log over them; with
+    ``cost="points"`` the one point-cloud kernel that evaluates the cost on
+    the fly.  The captured work contains no memset nodes and no
     host synchronisation.  Results are views of static buffers, overwritten
     by the next ``solve``.
     """
 
     def __init__(self, ns, ms, d, numItermax=1000, stopThr=1e-9, check_every=10,
-                 fallback_thr=0.0, waves=0, prefetch=0, occupancy=0, device=None):
+                 fallback_thr=0.0, waves=0, prefetch=0, occupancy=0, device=None,
+                 cost="dense"):
+        if cost not in ("dense", "points"):
+            raise ValueError("cost must be 'dense' or 'points'")
+        self.cost = cost
+        self.normalize = True
         self.device = device or _device.require_gpu()
         dev = self.device
         self.lib = _lib.load()
@@ -241,8 +391,11 @@ class RaggedGraph:
         tn, tm = int(self.ns.sum()), int(self.ms.sum())
         self.X = torch.zeros((tn, self.d), dtype=torch.float32, device=dev)
         self.Y = torch.zeros((tm, self.d), dtype=torch.float32, device=dev)
-        self.M = torch.empty(total, dtype=torch.float32, device=dev)
-        self.cmax = torch.empty(B, dtype=torch.float32, device=dev)
+        if cost == "points" and self.d > MAX_D_POINTS:
+            raise ValueError(f"the point-cloud solver supports d <= {MAX_D_POINTS}")
+        self.M = torch.empty(total if cost == "dense" else 1, dtype=torch.float32, device=dev)
+        self.cmax = torch.empty(B, dtype=torch.float32 if cost == "dense" else torch.float64,
+                                device=dev)
         self.reg_in = torch.ones(B, dtype=torch.float64, device=dev)
         self.reg_eff = torch.ones(B, dtype=torch.float64, device=dev)
         self.a = _weights(None, self.ns, dev)
@@ -269,9 +422,20 @@ class RaggedGraph:
             self._enqueue()
         torch.cuda.synchronize(dev)
 
+    @property
+    def batch(self):
+        return len(self.ns)
+
     def _enqueue(self):
         dev = self.device
         stream = ctypes.c_void_p(_device.stream_ptr(dev))
+        if self.cost == "points":
+            self.u.copy_(self.u0)
+            max_iter, check_every, stop, fb, _ = self.settings
+            _launch_points(self.lib, self, self.X, self.Y, self.reg_in, self.u, self.v,
+                           self.niter, self.status, self.e_last, self.e_first, self.cmax,
+                           self.order, max_iter, check_every, stop, fb, stream)
+            return
         rc = self.lib.potamd_sqeuclidean_ragged(
             len(self.ns), int(self.ns.max()), int(self.ms.max()), self.d, _ptr(self.X),
             _ptr(self.Y), _ptr(self.aoff), _ptr(self.boff), _ptr(self.ns_d), _ptr(self.ms_d),
@@ -301,9 +465,11 @@ class RaggedGraph:
         else:
             self.u0.copy_(u0)
         if order is None:
-            work = self.ns.astype(np.float64) * self.ms / regs
-            order = np.argsort(-work, kind="stable").astype(np.int32)
-        self.order.copy_(torch.as_tensor(np.asarray(order, dtype=np.int32)))
+            order = _points_order(self.ns, self.ms, regs)
+        else:
+            order = _check_order(order, len(self.ns))
+        self.order.copy_(torch.as_tensor(order))
         self.graph.replay()
         return RaggedResult(u=self.u, v=self.v, niter=self.niter, status=self.status,
-                            err_first=self.e_first, err_last=self.e_last)
+                            err_first=self.e_first, err_last=self.e_last,
+                            cmax=self.cmax if self.cost == "points" else None)

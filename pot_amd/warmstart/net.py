@@ -257,7 +257,8 @@ class WarmStartNet:
         if not (np.all(R.ns == n) and np.all(R.ms == m)):
             raise ValueError("the warm-start network expects problems of one size")
         regs_t = torch.as_tensor(np.asarray(regs, dtype=np.float64)).to(self.device)
+        cmax = R.cmax if R.cmax.dtype == torch.float32 else R.cmax.float()
         Sin, Ein, qn = self.inputs(X.reshape(-1, DX).float(), Y.reshape(-1, DX).float(), R.a,
-                                   R.b, regs_t, R.cmax, P, n, m)
+                                   R.b, regs_t, cmax, P, n, m)
         f = self.forward(Sin, Ein, qn, P, n, m)
         return (f.double().view(P, n) / regs_t[:, None]).reshape(-1)

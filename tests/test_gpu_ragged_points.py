@@ -1,0 +1,230 @@
+"""GPU parity of the point-cloud batch solver (cost computed on the fly).
+
+Each problem must behave like a lone ``sinkhorn_log`` call on
+``ot.dist(x, y) / M.max()`` evaluated in float64 from the same float32
+points: the plan within 1e-5 relative (entries >= 1e-30) at a fixed
+iteration count, the stop at the same check (one check of slack), the plan
+within 1e-4 of its largest entry at convergence, and the cost maximum that
+the kernel finds equal to the float64 maximum.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.sinkhorn as osk
+from oracle.utils import dist
+from pot_amd import problems, ragged
+
+pytestmark = pytest.mark.gpu
+
+
+def _cost(x, y, normalize=True):
+    M = dist(np.asarray(x, dtype=np.float32).astype(np.float64),
+             np.asarray(y, dtype=np.float32).astype(np.float64))
+    return M / M.max() if normalize else M
+
+
+def _plan(M, u, v, reg):
+    return np.exp(-M / reg + u[:, None] + v[None, :])
+
+
+def test_points_fixed_iterations_match_oracle():
+    # Uneven n, m (not multiples of 32), d < 16.
+    fam = problems.ragged_family(12, seed=321, n_range=(20, 300), d=5, kmin=1, kmax=3,
+                                 reg_range=(0.02, 0.2))
+    P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
+    it = 60
+    res = ragged.sinkhorn_log_ragged_points(P, numItermax=it, stopThr=0.0)
+    torch.cuda.synchronize()
+    print("fixed-iteration batch done", flush=True)
+    assert (res.niter.cpu().numpy() == it - 1).all()
+    assert (res.status.cpu().numpy() == 1).all()
+    cmax = res.cmax.cpu().numpy()
+    for p in range(P.batch):
+        Mraw = _cost(fam.xs[p], fam.ys[p], normalize=False)
+        assert abs(cmax[p] - Mraw.max()) <= 1e-12 * Mraw.max(), (p, cmax[p], Mraw.max())
+        M = Mraw / Mraw.max()
+        n, m = M.shape
+        reg = fam.regs[p]
+        a = np.full(n, 1.0 / n)
+        b = np.full(m, 1.0 / m)
+        Pr, lr = osk.sinkhorn_log(a, b, M, reg, numItermax=it, stopThr=0.0, log=True,
+                                  warn=False)
+        u, v = res.split(P, p)
+        Pl = _plan(M, u.cpu().numpy(), v.cpu().numpy(), reg)
+        mask = Pr >= 1e-30
+        rel = np.max(np.abs(Pl[mask] - Pr[mask]) / Pr[mask])
+        assert rel < 1e-5, (p, rel)
+        cost_rel = abs(np.sum(Pl * M) - np.sum(Pr * M)) / np.sum(Pr * M)
+        assert cost_rel < 1e-6, (p, cost_rel)
+        np.testing.assert_allclose(res.err_first[p].item(), lr["err"][0], rtol=1e-4)
+
+
+def test_points_converged_matches_oracle_small_reg():
+    # The warm-start family's regime: GMM clouds, d = 16, reg down to 5e-3.
+    fam = problems.ragged_family(6, seed=77, n_range=(64, 320), d=16, kmin=1, kmax=8,
+                                 reg_range=(5e-3, 5e-2))
+    P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
+    res = ragged.sinkhorn_log_ragged_points(P, numItermax=3000, stopThr=1e-6)
+    torch.cuda.synchronize()
+    print("converged batch done", flush=True)
+    niter = res.niter.cpu().numpy()
+    assert ((res.status.cpu().numpy() & 3) == 0).all()
+    for p in range(P.batch):
+        M = _cost(fam.xs[p], fam.ys[p])
+        n, m = M.shape
+        reg = fam.regs[p]
+        a = np.full(n, 1.0 / n)
+        b = np.full(m, 1.0 / m)
+        Pr, lr = osk.sinkhorn_log(a, b, M, reg, numItermax=3000, stopThr=1e-6, log=True,
+                                  warn=False)
+        assert abs(int(niter[p]) - lr["niter"]) <= 10, (p, niter[p], lr["niter"])
+        u, v = res.split(P, p)
+        Pl = _plan(M, u.cpu().numpy(), v.cpu().numpy(), reg)
+        assert np.max(np.abs(Pl - Pr)) / Pr.max() < 1e-4, p
+        assert np.linalg.norm(Pl.sum(0) - b) < 2e-6
+        cost_rel = abs(np.sum(Pl * M) - np.sum(Pr * M)) / np.sum(Pr * M)
+        assert cost_rel < 1e-5, (p, cost_rel)
+
+
+def test_points_full_size_problem_matches_oracle():
+    fam = problems.config5(seed=5, count=1)
+    x, y, reg = fam.xs[0], fam.ys[0], float(fam.regs[0])
+    P = ragged.points_from_clouds([x], [y], [reg])
+    it = 30
+    res = ragged.sinkhorn_log_ragged_points(P, numItermax=it, stopThr=0.0)
+    torch.cuda.synchronize()
+    print("1024 x 1024 problem done", flush=True)
+    M = _cost(x, y)
+    n, m = M.shape
+    Pr = osk.sinkhorn_log(np.full(n, 1 / n), np.full(m, 1 / m), M, reg, numItermax=it,
+                          stopThr=0.0, warn=False)
+    Pl = _plan(M, res.u.cpu().numpy(), res.v.cpu().numpy(), reg)
+    mask = Pr >= 1e-30
+    rel = np.max(np.abs(Pl[mask] - Pr[mask]) / Pr[mask])
+    assert rel < 1e-5, rel
+
+
+def test_points_agree_with_materialised_batch():
+    fam = problems.ragged_family(24, seed=11, n_range=(64, 1024), d=16, kmin=1, kmax=4,
+                                 reg_range=(5e-3, 5e-2))
+    P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
+    R = ragged.from_clouds(fam.xs, fam.ys, fam.regs)
+    rp = ragged.sinkhorn_log_ragged_points(P, numItermax=2000, stopThr=1e-6, check_every=1)
+    rd = ragged.sinkhorn_log_ragged(R, numItermax=2000, stopThr=1e-6, check_every=1)
+    torch.cuda.synchronize()
+    print("points vs materialised done", flush=True)
+    assert ((rp.status.cpu().numpy() & 3) == 0).all()
+    assert ((rd.status.cpu().numpy() & 3) == 0).all()
+    for p in range(P.batch):
+        M = _cost(fam.xs[p], fam.ys[p])
+        reg = fam.regs[p]
+        up, vp = rp.split(P, p)
+        ud, vd = rd.split(R, p)
+        Pp = _plan(M, up.cpu().numpy(), vp.cpu().numpy(), reg)
+        Pd = _plan(M, ud.cpu().numpy(), vd.cpu().numpy(), reg)
+        assert np.max(np.abs(Pp - Pd)) / Pd.max() < 1e-4, p
+
+
+def test_points_warm_start_fallback_and_edge_cases():
+    rng = np.random.default_rng(3)
+    shapes = [(1, 7, 3), (13, 1, 3), (40, 33, 1), (200, 1024, 16), (1024, 65, 8)]
+    xs = [rng.standard_normal((n, d)).astype(np.float32) for n, m, d in shapes]
+    ys = [(rng.standard_normal((m, d)) + 0.3).astype(np.float32) for n, m, d in shapes]
+    # One d for the whole batch: pad the clouds with zero coordinates.
+    d = max(s[2] for s in shapes)
+    xs = [np.pad(x, ((0, 0), (0, d - x.shape[1]))) for x in xs]
+    ys = [np.pad(y, ((0, 0), (0, d - y.shape[1]))) for y in ys]
+    regs = [0.1, 0.05, 0.2, 0.02, 0.05]
+    a = [np.full(n, 1.0 / n) for n, _, _ in shapes]
+    b = [np.full(m, 1.0 / m) for _, m, _ in shapes]
+    # zero weights in one problem
+    a[3] = a[3].copy()
+    a[3][:10] = 0.0
+    a[3] /= a[3].sum()
+    b[3] = b[3].copy()
+    b[3][5:9] = 0.0
+    b[3] /= b[3].sum()
+    P = ragged.points_from_clouds(xs, ys, regs, a=a, b=b)
+    res = ragged.sinkhorn_log_ragged_points(P, numItermax=5000, stopThr=1e-6)
+    torch.cuda.synchronize()
+    print("edge-case batch done", flush=True)
+    assert ((res.status.cpu().numpy() & 3) == 0).all()
+    for p in range(P.batch):
+        M = _cost(xs[p], ys[p])
+        Pr = osk.sinkhorn_log(a[p], b[p], M, regs[p], numItermax=5000, stopThr=1e-6,
+                              warn=False)
+        u, v = res.split(P, p)
+        Pl = _plan(M, u.cpu().numpy(), v.cpu().numpy(), regs[p])
+        assert np.max(np.abs(Pl - Pr)) / Pr.max() < 1e-4, p
+    u_p, _ = res.split(P, 3)
+    assert np.isneginf(u_p.cpu().numpy()[:10]).all()
+    # Warm start from the converged potentials: stops at the first check.
+    res2 = ragged.sinkhorn_log_ragged_points(P, u0=res.u, numItermax=5000, stopThr=1e-6)
+    # A wild warm start with the fallback enabled restarts cold.
+    wild = torch.as_tensor(20.0 * rng.standard_normal(res.u.numel()), device=res.u.device)
+    res3 = ragged.sinkhorn_log_ragged_points(P, u0=wild, numItermax=5000, stopThr=1e-6,
+                                             fallback_thr=1e-2)
+    torch.cuda.synchronize()
+    print("warm-start batches done", flush=True)
+    assert (res2.niter.cpu().numpy() == 0).all()
+    st3 = res3.status.cpu().numpy()
+    assert ((st3 & 3) == 0).all()
+    # With a single row or column the first sweep is exact whatever u0 is, so
+    # only problems 2-4 can reject the wild start; after the restart they
+    # repeat the cold solve exactly.
+    assert ((st3[2:] & 4) > 0).all()
+    np.testing.assert_array_equal(res3.niter.cpu().numpy()[2:], res.niter.cpu().numpy()[2:])
+    for p in (2, 3, 4):
+        np.testing.assert_array_equal(res3.split(P, p)[0].cpu().numpy(),
+                                      res.split(P, p)[0].cpu().numpy())
+
+
+def test_points_unnormalised_and_out_of_range_reg():
+    rng = np.random.default_rng(4)
+    xs = [rng.standard_normal((50, 4)).astype(np.float32) for _ in range(3)]
+    ys = [(rng.standard_normal((60, 4)) + 1.0).astype(np.float32) for _ in range(3)]
+    regs = [2.0, 1e-14, 5.0]
+    P = ragged.points_from_clouds(xs, ys, regs, normalize=False)
+    res = ragged.sinkhorn_log_ragged_points(P, numItermax=3000, stopThr=1e-7)
+    torch.cuda.synchronize()
+    print("unnormalised batch done", flush=True)
+    st = res.status.cpu().numpy()
+    assert st[1] == 2 and (st[[0, 2]] == 0).all()
+    u1, _ = res.split(P, 1)
+    assert np.isnan(u1.cpu().numpy()).all()
+    np.testing.assert_array_equal(res.cmax.cpu().numpy(), [1.0, 1.0, 1.0])
+    for p in (0, 2):
+        M = _cost(xs[p], ys[p], normalize=False)
+        Pr = osk.sinkhorn_log(np.full(50, 1 / 50), np.full(60, 1 / 60), M, regs[p],
+                              numItermax=3000, stopThr=1e-7, warn=False)
+        u, v = res.split(P, p)
+        Pl = _plan(M, u.cpu().numpy(), v.cpu().numpy(), regs[p])
+        assert np.max(np.abs(Pl - Pr)) / Pr.max() < 1e-4, p
+
+
+def test_points_graph_replays_match_eager():
+    sizes = problems.ragged_family(10, seed=5, n_range=(40, 400), d=6)
+    ns = [len(x) for x in sizes.xs]
+    ms = [len(y) for y in sizes.ys]
+    G = ragged.RaggedGraph(ns, ms, 6, numItermax=500, stopThr=1e-7, check_every=10,
+                           cost="points")
+    for rep in range(3):
+        rng = np.random.default_rng(200 + rep)
+        xs = [rng.standard_normal((n, 6)).astype(np.float32) for n in ns]
+        ys = [(rng.standard_normal((m, 6)) + 0.5).astype(np.float32) for m in ms]
+        regs = np.exp(rng.uniform(np.log(0.02), np.log(0.2), size=len(ns)))
+        P = ragged.points_from_clouds(xs, ys, regs)
+        ref = ragged.sinkhorn_log_ragged_points(P, numItermax=500, stopThr=1e-7,
+                                                check_every=10)
+        torch.cuda.synchronize()
+        ref_u, ref_v = ref.u.cpu().numpy(), ref.v.cpu().numpy()
+        ref_n = ref.niter.cpu().numpy()
+        out = G.solve(xs, ys, regs)
+        torch.cuda.synchronize()
+        print("graph replay", rep, "done", flush=True)
+        assert np.array_equal(out.niter.cpu().numpy(), ref_n)
+        assert np.array_equal(out.u.cpu().numpy(), ref_u)
+        assert np.array_equal(out.v.cpu().numpy(), ref_v)
+        _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
