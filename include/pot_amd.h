@@ -192,6 +192,7 @@ int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int max_m, const 
  *    potentials.
  * cmax: per-problem cost maximum (device double[B]): written with normalize
  *    0 (1.0) or 1, read with 2; may be NULL unless normalize == 2.
+ * waves: wavefronts per problem, 8 or 16 (0 = default 8).
  */
 int potamd_sinkhorn_log_batch_points(int batch, int max_n, int max_m, int d, const float* X,
                                      const float* Y, const int64_t* xoff, const int64_t* yoff,
@@ -200,7 +201,7 @@ int potamd_sinkhorn_log_batch_points(int batch, int max_n, int max_m, int d, con
                                      double* v, int max_iter, int check_every, double stop_thr,
                                      double fallback_thr, const int* order, int* niter,
                                      int* status, double* err_last, double* err_first,
-                                     double* cmax, void* stream);
+                                     double* cmax, int waves, void* stream);
 
 /* Per-problem maximum of |x_r - y_c|^2 (the M.max() of ot.dist(x, y) /
  * M.max()), the same device search as potamd_sinkhorn_log_batch_points with
@@ -242,6 +243,20 @@ int potamd_gemm_bf16(int M, int N, int K, const void* A, int64_t lda, int transA
 int potamd_mlp_op(int op, int rows, int seg_len, int P, int k, int n, int64_t count, int flag,
                   double scalar, const float* hp, const void* x, const void* y, const void* z,
                   const void* w, void* out, void* out2, void* stream);
+
+/* Fused inference forward of one two-layer chain of the warm-start network
+ * (bf16 MFMA, fp32 accumulation, leaky ReLU slope 0.01; hidden width 256,
+ * input width 32):  A1 = act(inp W1 + bias1), A2 = act(A1 W2 + b2) (A2
+ * rounded to bf16), then
+ *   mode 0: out[s][:] = mean of A2 over the rows of segment s (bias1 = b1[256]);
+ *   mode 1: out[r] = A2[r] . w3 + b3[0] + qn[r] (bias1 = rowvec[segment][256],
+ *           qn may be NULL).
+ * inp: rows x 32 bf16; W1: 32 x 256 and W2: 256 x 256 bf16, row-major.
+ * seg_len must be a multiple of 64 and divide rows.
+ */
+int potamd_mlp_fused(int mode, int rows, int seg_len, const void* inp, const void* W1,
+                     const float* bias1, const void* W2, const float* b2, const float* w3,
+                     const float* b3, const float* qn, float* out, void* stream);
 
 /* ---- Online cost: squared Euclidean cost of two point clouds, never stored ----
  *

@@ -47,7 +47,8 @@ SIGNATURES = {
         c_vp, c_vp, c_vp, c_vp, c_vp, c_int,     # xoff, yoff, ns, ms, regs, normalize
         c_vp, c_vp, c_vp, c_vp,                  # a, b, u, v
         c_int, c_int, c_dbl, c_dbl, c_vp,        # max_iter, check_every, stop, fallback, order
-        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),    # niter, status, err_last, err_first, cmax, stream
+        c_vp, c_vp, c_vp, c_vp, c_vp,            # niter, status, err_last, err_first, cmax
+        c_int, c_vp]),                           # waves, stream
     "potamd_ragged_points_cmax": (c_int, [
         c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "potamd_sqeuclidean_ragged": (c_int, [
@@ -56,6 +57,8 @@ SIGNATURES = {
     "potamd_gemm_bf16": (c_int, [
         c_int, c_int, c_int, c_vp, c_i64, c_int, c_vp, c_i64, c_int, c_vp, c_i64,
         c_int, ctypes.c_float, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_int, c_vp]),
+    "potamd_mlp_fused": (c_int, [
+        c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "potamd_mlp_op": (c_int, [
         c_int, c_int, c_int, c_int, c_int, c_int, c_i64, c_int, c_dbl, c_vp,
         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

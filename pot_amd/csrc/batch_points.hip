@@ -46,12 +46,10 @@
 namespace potamd {
 namespace {
 
-constexpr int kPW = 8;                       // wavefronts per problem
-constexpr int kPT = kPW * kWave;             // threads per problem
+constexpr int kPWDefault = 8;                // wavefronts per problem (8 or 16)
 constexpr int kPMax = 1024;                  // largest n and m
 constexpr int kPD = 16;                      // coordinates per point (zero padded)
 constexpr int kPBlocks = kPMax / 32;         // row blocks of 32
-constexpr int kPSlots = kPMax / (32 * kPW);  // column tiles per wave
 constexpr float kMaxOperand = 16384.f;       // largest |X|, |Y| accepted (f16 range)
 constexpr unsigned kOnes = 0x3F803F80u;      // two bf16 1.0
 
@@ -169,6 +167,20 @@ __device__ __forceinline__ f32x16 mfma_b(u32x4 a, u32x4 b, f32x16 c) {
                                                  __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
+// x[l] + x[l ^ 32] in every lane (the two wave halves hold the same row of
+// an MFMA tile), through v_permlane32_swap; the operands are added in the
+// same order in both halves, so the result is identical there.
+__device__ __forceinline__ float halves_sum(float x) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false,
+                                                  false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float halves_max(float x) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false,
+                                                  false);
+  return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+
 // Reduction over the 32 lanes of each wave half (the lanes that share a
 // column of an MFMA tile): four row-local DPP steps, then row 0 into row 1
 // and row 2 into row 3 (row_bcast:15).  Lanes 16-31 and 48-63 end with the
@@ -186,6 +198,7 @@ __device__ __forceinline__ float half_reduce(float x) {
   return op(x, y);
 }
 
+template <int PW>
 struct PointsLds {
   u32x4 rop[kPBlocks][2][kWave];  // row operands: [block][hi, lo][lane]
   u32x4 rks[kPMax];               // row K slots
@@ -193,23 +206,25 @@ struct PointsLds {
   double xn[kPMax];               // s |x_r|^2
   double vsh[kPMax];              // column potentials (base 2)
   double yn[kPMax];               // s |y_c|^2
+  double lbs[kPMax];              // log2 b_c (-inf where b_c = 0)
   float af[kPMax];                // row weights
   float sh[kPMax];                // shifted row sums (prologue: column maxima)
   float colx[kPMax];              // column sums
-  float rowpart[2][kPW][32];
-  float rowmax[kPW][32];
+  float rowpart[2][PW][32];
+  float rowmax[PW][32];
   double cen[kPD];
-  double red[kPW];
-  float redf[kPW];
+  double red[PW];
+  float redf[PW];
   int flag;
 };
 
 // Everything after the centring, for problems whose column tiles take NS
 // slots per wave (NS = ceil(ceil(m / 32) / 8)); slots past m hold padding
 // columns whose potential is -inf, so they add nothing.
-template <int NS>
-__device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, int p, int n,
+template <int PW, int NS>
+__device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>& L, int p, int n,
                                              int m, double ext) {
+  constexpr int PT = PW * kWave;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int h = lane >> 5, l32 = lane & 31;
   const int d = A.d;
@@ -222,17 +237,17 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, 
   double* vp = A.v + yo;
   const int nblk = (n + 31) / 32;
   // Tile j of this wave covers columns 32 * (wave + 8 j) + [0, 32).
-  auto col_of = [&](int j) { return 32 * (wave + kPW * j) + l32; };
+  auto col_of = [&](int j) { return 32 * (wave + PW * j) + l32; };
   // Column of value i of tile j in the MFMA output layout.
   auto out_col = [&](int j, int i) {
-    return 32 * (wave + kPW * j) + 8 * (i / 4) + 4 * h + (i % 4);
+    return 32 * (wave + PW * j) + 8 * (i / 4) + 4 * h + (i % 4);
   };
 
   // Operand packing: rows into rop / xn, this wave's columns into registers / yn.
   u32x4 ahi[NS], alo[NS], ak[NS];
   auto pack = [&](double sc) -> float {
     float mx = 0.f;
-    for (int q = tid; q < nblk * kWave; q += kPT) {
+    for (int q = tid; q < nblk * kWave; q += PT) {
       const int blk = q >> 6, ln = q & 63, r = blk * 32 + (ln & 31), hh = ln >> 5;
       u32x4 hi4, lo4;
       double nh = 0.0;
@@ -263,9 +278,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, 
   // D[c][r] tiles of this wave's columns against row block blk; the small
   // products first, the potential last.
   f32x16 e[NS];
-  auto tiles = [&](int blk, u32x4 bk) {
-    const u32x4 bh = L.rop[blk][0][lane];
-    const u32x4 bl = L.rop[blk][1][lane];
+  auto tiles_ops = [&](u32x4 bh, u32x4 bl, u32x4 bk) {
     const f32x16 z = {};
 #pragma unroll
     for (int j = 0; j < NS; ++j) e[j] = mfma_h(alo[j], bh, z);
@@ -276,6 +289,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, 
 #pragma unroll
     for (int j = 0; j < NS; ++j) e[j] = mfma_b(ak[j], bk, e[j]);
   };
+  auto tiles = [&](int blk, u32x4 bk) { tiles_ops(L.rop[blk][0][lane], L.rop[blk][1][lane], bk); };
   // e <- exp2(e); returns the lane's row sum over this wave's columns (both
   // halves of the wave hold the same row), one partial per tile.
   auto exp_rowsum = [&]() {
@@ -292,7 +306,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, 
     float sl = part[0];
 #pragma unroll
     for (int j = 1; j < NS; ++j) sl += part[j];
-    return sl + __shfl_xor(sl, 32, kWave);
+    return halves_sum(sl);
   };
   auto row_kslot = [&](int blk) {
     const u32x4 k = L.rks[blk * 32 + l32];
@@ -312,7 +326,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, 
       }
       if (l32 >= 16) {
         const int i = l32 - 16;
-        dst[32 * (wave + kPW * j) + 8 * (i / 4) + 4 * h + (i % 4)] = w;
+        dst[32 * (wave + PW * j) + 8 * (i / 4) + 4 * h + (i % 4)] = w;
       }
     }
   };
@@ -325,7 +339,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, 
     __syncthreads();
     // Padding is +inf so that it never wins the minimum of -cost.
     set_col_kslots([&](int c) { return c < m ? -L.yn[c] : __builtin_inf(); });
-    for (int r = tid; r < nblk * 32; r += kPT)
+    for (int r = tid; r < nblk * 32; r += PT)
       L.rks[r] = kslot_row(r < n ? L.xn[r] : -__builtin_inf());
     __syncthreads();
     float best = __builtin_inff();
@@ -357,7 +371,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, 
     if (lane == 0) L.red[wave] = mexact;
     __syncthreads();
     cmax = 0.0;
-    for (int w = 0; w < kPW; ++w) cmax = fmax(cmax, L.red[w]);
+    for (int w = 0; w < PW; ++w) cmax = fmax(cmax, L.red[w]);
     __syncthreads();
     if (A.max_only) {
       if (tid == 0) A.cmax[p] = cmax;
@@ -372,12 +386,12 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, 
   }
   __syncthreads();
   float allmax = 0.f;
-  for (int w = 0; w < kPW; ++w) allmax = fmaxf(allmax, L.redf[w]);
+  for (int w = 0; w < PW; ++w) allmax = fmaxf(allmax, L.redf[w]);
   if (!(allmax <= kMaxOperand) || !(cmax > 0.0) || !(reg > 0.0)) {
     // Outside the operand range of the f16 MFMA path (reg far below the
     // cost scale) or a degenerate cost: report instead of solving.
-    for (int i = tid; i < n; i += kPT) up[i] = __builtin_nan("");
-    for (int j = tid; j < m; j += kPT) vp[j] = __builtin_nan("");
+    for (int i = tid; i < n; i += PT) up[i] = __builtin_nan("");
+    for (int j = tid; j < m; j += PT) vp[j] = __builtin_nan("");
     if (tid == 0) {
       A.niter[p] = 0;
       A.status[p] = kNumerical;
@@ -393,7 +407,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds& L, 
 
 restart:
   // ---- prologue: v_1 = log2 b - LSE2_r(x_rc + u2_r), a max pass then a sum pass ----
-  for (int r = tid; r < nblk * 32; r += kPT) {
+  for (int r = tid; r < nblk * 32; r += PT) {
     const double u2 = (r < n && !cold) ? up[r] * kLog2e : 0.0;
     L.rks[r] = kslot_row(r < n ? L.xn[r] - u2 : __builtin_inf());
   }
@@ -430,13 +444,14 @@ restart:
     const int c = col_of(j);
     if (h == 0 && c < m) {
       const double bj = bp[c];
+      L.lbs[c] = bj > 0.0 ? log2(bj) : -__builtin_inf();
       const double g = (double)L.sh[c];
       L.vsh[c] = (bj > 0.0 && __builtin_isfinite(g)) ? log2(bj) - (g + log2((double)L.colx[c]))
                                                      : -__builtin_inf();
     }
   }
   set_col_kslots([&](int c) { return c < m ? L.vsh[c] - L.yn[c] : -__builtin_inf(); });
-  for (int r = tid; r < nblk * 32; r += kPT) {
+  for (int r = tid; r < nblk * 32; r += PT) {
     double c2 = __builtin_inf();
     if (r < n && L.af[r] > 0.f) {
       const double u2 = cold ? 0.0 : up[r] * kLog2e;
@@ -451,16 +466,24 @@ restart:
     // ---- fused sweep: row sums (u update) and column sums of P(u_new, v) ----
 #pragma unroll
     for (int j = 0; j < NS; ++j) acc[j] = f32x16{};
+    // The row operands of the next block are read one block ahead, so their
+    // LDS latency overlaps this block's work.
+    u32x4 nbh = L.rop[0][0][lane], nbl = L.rop[0][1][lane], nbk = row_kslot(0);
     for (int blk = 0; blk < nblk; ++blk) {
-      u32x4 bk = row_kslot(blk);
-      tiles(blk, bk);
+      u32x4 bk = nbk;
+      tiles_ops(nbh, nbl, bk);
+      if (blk + 1 < nblk) {
+        nbh = L.rop[blk + 1][0][lane];
+        nbl = L.rop[blk + 1][1][lane];
+        nbk = row_kslot(blk + 1);
+      }
       float sl = exp_rowsum();
       const int buf = blk & 1;
       if (h == 0) L.rowpart[buf][wave][l32] = sl;
       __syncthreads();
       float S = 0.f;
 #pragma unroll
-      for (int w = 0; w < kPW; ++w) S += L.rowpart[buf][w][l32];
+      for (int w = 0; w < PW; ++w) S += L.rowpart[buf][w][l32];
       const int r = blk * 32 + l32;
       const float ar = L.af[r];
       const bool bad = ar > 0.f && !in_safe_range((double)S);
@@ -472,12 +495,12 @@ restart:
         for (int j = 0; j < NS; ++j)
 #pragma unroll
           for (int i = 0; i < 16; ++i) mx = fmaxf(mx, e[j][i]);
-        mx = fmaxf(mx, __shfl_xor(mx, 32, kWave));
+        mx = halves_max(mx);
         if (h == 0) L.rowmax[wave][l32] = mx;
         __syncthreads();
         float G = -__builtin_inff();
 #pragma unroll
-        for (int w = 0; w < kPW; ++w) G = fmaxf(G, L.rowmax[w][l32]);
+        for (int w = 0; w < PW; ++w) G = fmaxf(G, L.rowmax[w][l32]);
         double c2 = L.c2s[r];
         if (bad && __builtin_isfinite(G)) {
           c2 += (double)G;
@@ -490,7 +513,7 @@ restart:
         __syncthreads();
         S = 0.f;
 #pragma unroll
-        for (int w = 0; w < kPW; ++w) S += L.rowpart[buf][w][l32];
+        for (int w = 0; w < PW; ++w) S += L.rowpart[buf][w][l32];
         if (wave == 0 && h == 0 && bad) {
           L.c2s[r] = c2;  // the shift actually used
           L.rks[r] = bk;
@@ -507,9 +530,10 @@ restart:
     __syncthreads();
 
     // ---- row potentials, marginal error, stop decision ----
-    for (int r = tid; r < n; r += kPT) {
+    for (int r = tid; r < n; r += PT) {
       if (L.af[r] > 0.f) {
-        const double c = L.c2s[r] + log2((double)L.sh[r]);
+        // sh[r] lies in [2^-60, 2^60] (the safe range of the sweep).
+        const double c = L.c2s[r] + (double)__builtin_amdgcn_logf(L.sh[r]);
         L.c2s[r] = c;
         L.rks[r] = kslot_row(c + L.xn[r]);
       }
@@ -529,7 +553,7 @@ restart:
     if (tid == 0) {
       double t = 0.0;
 #pragma unroll
-      for (int w = 0; w < kPW; ++w) t += L.red[w];
+      for (int w = 0; w < PW; ++w) t += L.red[w];
       const double err = sqrt(t);
       int stop = 0;
       if (ii % A.check_every == 0) {
@@ -562,9 +586,11 @@ restart:
     for (int j = 0; j < NS; ++j) {
       const int c = col_of(j);
       if (h == 0 && c < m) {
-        const double bj = bp[c];
-        const double vn =
-            bj > 0.0 ? L.vsh[c] + log2(bj) - log2((double)L.colx[c]) : -__builtin_inf();
+        const float cs = L.colx[c];
+        const double lcs = cs >= 1.1754944e-38f ? (double)__builtin_amdgcn_logf(cs)
+                                                : log2((double)cs);
+        const double vn = L.lbs[c] > -__builtin_inf() ? L.vsh[c] + L.lbs[c] - lcs
+                                                      : -__builtin_inf();
         L.vsh[c] = vn;
         ak[j] = kslot_col(vn - L.yn[c]);
       }
@@ -574,9 +600,9 @@ restart:
 
   // ---- outputs (natural-log potentials) ----
   __syncthreads();
-  for (int i = tid; i < n; i += kPT)
+  for (int i = tid; i < n; i += PT)
     up[i] = L.af[i] > 0.f ? (log2(ap[i]) - L.c2s[i]) * kLn2 : -__builtin_inf();
-  for (int j = tid; j < m; j += kPT) vp[j] = L.vsh[j] * kLn2;
+  for (int j = tid; j < m; j += PT) vp[j] = L.vsh[j] * kLn2;
   if (tid == 0) {
     A.niter[p] = niter;
     A.status[p] = status | (restarted ? 4 : 0);
@@ -586,8 +612,11 @@ restart:
   }
 }
 
-__global__ __launch_bounds__(kPT) void batch_points_kernel(PointsArgs A) {
-  __shared__ PointsLds L;
+template <int PW>
+__global__ __launch_bounds__(PW * kWave) void batch_points_kernel(PointsArgs A) {
+  constexpr int PT = PW * kWave;
+  constexpr int NG = PT / 16;  // point groups of the centring reduction
+  __shared__ PointsLds<PW> L;
   const int p = A.order ? A.order[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
   const int n = A.ns[p], m = A.ms[p], d = A.d;
@@ -606,24 +635,24 @@ __global__ __launch_bounds__(kPT) void batch_points_kernel(PointsArgs A) {
   const float* Yp = A.Y + A.yoff[p] * d;
   if (!A.max_only) {
     const double* ap = A.a + A.xoff[p];
-    for (int i = tid; i < kPMax; i += kPT) L.af[i] = i < n ? (float)ap[i] : 0.f;
+    for (int i = tid; i < kPMax; i += PT) L.af[i] = i < n ? (float)ap[i] : 0.f;
   }
 
   // ---- centre (midpoint of the centroids) and coordinate extent ----
   double ext = 0.0;  // largest |coordinate - centre| over both clouds
   {
     double* scr = reinterpret_cast<double*>(&L.rop[0][0][0]);
-    const int k = tid & 15, g = tid >> 4;
+    const int k = tid & 15, g = tid >> 4;  // g < NG
     double sx = 0.0, sy = 0.0;
     float lo = __builtin_inff(), hi = -__builtin_inff();
     if (k < d) {
-      for (int i = g; i < n; i += 32) {
+      for (int i = g; i < n; i += NG) {
         const float x = Xp[(int64_t)i * d + k];
         sx += x;
         lo = fminf(lo, x);
         hi = fmaxf(hi, x);
       }
-      for (int i = g; i < m; i += 32) {
+      for (int i = g; i < m; i += NG) {
         const float y = Yp[(int64_t)i * d + k];
         sy += y;
         lo = fminf(lo, y);
@@ -631,35 +660,42 @@ __global__ __launch_bounds__(kPT) void batch_points_kernel(PointsArgs A) {
       }
     }
     scr[tid] = sx;
-    scr[kPT + tid] = sy;
-    scr[2 * kPT + tid] = lo;
-    scr[3 * kPT + tid] = hi;
+    scr[PT + tid] = sy;
+    scr[2 * PT + tid] = lo;
+    scr[3 * PT + tid] = hi;
     __syncthreads();
     if (tid < kPD) {
       double cx = 0.0, cy = 0.0, l = __builtin_inf(), u = -__builtin_inf();
-      for (int gg = 0; gg < 32; ++gg) {
+      for (int gg = 0; gg < NG; ++gg) {
         cx += scr[gg * 16 + tid];
-        cy += scr[kPT + gg * 16 + tid];
-        l = fmin(l, scr[2 * kPT + gg * 16 + tid]);
-        u = fmax(u, scr[3 * kPT + gg * 16 + tid]);
+        cy += scr[PT + gg * 16 + tid];
+        l = fmin(l, scr[2 * PT + gg * 16 + tid]);
+        u = fmax(u, scr[3 * PT + gg * 16 + tid]);
       }
       const double c = tid < d ? 0.5 * (cx / n + cy / m) : 0.0;
       L.cen[tid] = c;
-      scr[4 * kPT + tid] = tid < d ? fmax(u - c, c - l) : 0.0;
+      scr[4 * PT + tid] = tid < d ? fmax(u - c, c - l) : 0.0;
     }
     __syncthreads();
-    for (int kk = 0; kk < kPD; ++kk) ext = fmax(ext, scr[4 * kPT + kk]);
+    for (int kk = 0; kk < kPD; ++kk) ext = fmax(ext, scr[4 * PT + kk]);
     __syncthreads();  // the scratch is the row-operand area, packed next
   }
-  const int ns = ((m + 31) / 32 + kPW - 1) / kPW;
-  if (ns <= 1)
-    solve_points<1>(A, L, p, n, m, ext);
-  else if (ns == 2)
-    solve_points<2>(A, L, p, n, m, ext);
-  else if (ns == 3)
-    solve_points<3>(A, L, p, n, m, ext);
-  else
-    solve_points<4>(A, L, p, n, m, ext);
+  const int ns = ((m + 31) / 32 + PW - 1) / PW;
+  if constexpr (PW == 16) {
+    if (ns <= 1)
+      solve_points<PW, 1>(A, L, p, n, m, ext);
+    else
+      solve_points<PW, 2>(A, L, p, n, m, ext);
+  } else {
+    if (ns <= 1)
+      solve_points<PW, 1>(A, L, p, n, m, ext);
+    else if (ns == 2)
+      solve_points<PW, 2>(A, L, p, n, m, ext);
+    else if (ns == 3)
+      solve_points<PW, 3>(A, L, p, n, m, ext);
+    else
+      solve_points<PW, 4>(A, L, p, n, m, ext);
+  }
 }
 
 }  // namespace
@@ -672,12 +708,14 @@ extern "C" int potamd_sinkhorn_log_batch_points(
     const int64_t* yoff, const int* ns, const int* ms, const double* regs, int normalize,
     const double* a, const double* b, double* u, double* v, int max_iter, int check_every,
     double stop_thr, double fallback_thr, const int* order, int* niter, int* status,
-    double* err_last, double* err_first, double* cmax, void* stream) {
+    double* err_last, double* err_first, double* cmax, int waves, void* stream) {
   if (batch < 1 || max_n < 1 || max_m < 1 || d < 1 || !X || !Y || !xoff || !yoff || !ns ||
       !ms || !regs || !a || !b || !u || !v || max_iter < 1 || !niter || !status ||
       check_every < 0 || normalize < 0 || normalize > 2 || (normalize == 2 && !cmax))
     return POTAMD_EINVAL;
   if (max_n > kPMax || max_m > kPMax || d > kPD) return POTAMD_EUNSUPPORTED;
+  if (waves == 0) waves = kPWDefault;
+  if (waves != 8 && waves != 16) return POTAMD_EINVAL;
   PointsArgs pa;
   memset(&pa, 0, sizeof(pa));
   pa.X = X;
@@ -703,8 +741,12 @@ extern "C" int potamd_sinkhorn_log_batch_points(
   pa.err_last = err_last;
   pa.err_first = err_first;
   pa.cmax = cmax;
-  hipLaunchKernelGGL(batch_points_kernel, dim3(batch), dim3(kPT), 0,
-                     static_cast<hipStream_t>(stream), pa);
+  if (waves == 16)
+    hipLaunchKernelGGL(batch_points_kernel<16>, dim3(batch), dim3(16 * kWave), 0,
+                       static_cast<hipStream_t>(stream), pa);
+  else
+    hipLaunchKernelGGL(batch_points_kernel<8>, dim3(batch), dim3(8 * kWave), 0,
+                       static_cast<hipStream_t>(stream), pa);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -729,7 +771,7 @@ extern "C" int potamd_ragged_points_cmax(int batch, int max_n, int max_m, int d,
   pa.normalize = 1;
   pa.max_only = 1;
   pa.cmax = cmax;
-  hipLaunchKernelGGL(batch_points_kernel, dim3(batch), dim3(kPT), 0,
+  hipLaunchKernelGGL(batch_points_kernel<kPWDefault>, dim3(batch), dim3(kPWDefault * kWave), 0,
                      static_cast<hipStream_t>(stream), pa);
   HIP_TRY(hipGetLastError());
   return 0;

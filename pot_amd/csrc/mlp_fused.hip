@@ -1,0 +1,251 @@
+// Fused two-layer forward of the warm-start network for inference: the
+// 256-wide hidden activations never leave the chip.  No counterpart in the
+// reference.
+//
+// Both modes run, on blocks of 64 input rows (32 bf16 features each),
+//     A1 = act(inp W1 + bias1)           (64 x 256, bf16 in LDS)
+//     A2 = act(A1 W2 + b2)               (64 x 256, rounded to bf16)
+// with act the leaky ReLU (slope 0.01), and then
+//  * mode 0 (encoder): the mean of A2 over each segment of seg_len rows
+//    (bias1 = b1[256]); out = (rows / seg_len) x 256 means;
+//  * mode 1 (head): out[r] = A2[r] . w3 + b3 + qn[r], with bias1 the
+//    per-segment vector rowvec[r / seg_len][256].
+// These are the layers that pot_amd/warmstart/net.py otherwise runs as bf16
+// GEMMs with the same epilogues (bias / broadcast, leaky ReLU, bf16 store),
+// a segment mean and a head kernel.
+//
+// Layout: a 256-thread workgroup (one wave per SIMD) walks whole segments;
+// wave w owns hidden columns 64 w .. 64 w + 63 of both layers and keeps its
+// slices of W1 (32 x 64) and W2 (256 x 64) in registers for the whole
+// launch.  Layer 1 is computed transposed (D[j][r]) so each lane holds four
+// consecutive columns of one row and writes them to LDS with one 8-byte
+// store; layer 2 reads A1 from LDS (16 bytes per lane per MFMA K step).  In
+// mode 0 layer 2 gives D[r][j], so a lane owns a column and the segment sum
+// is a per-lane accumulation; in mode 1 it gives D[j][r] (operands swapped),
+// so a lane owns a row and the head's dot product is per lane.  A1 is double
+// buffered in LDS: one barrier per block (two in mode 1).
+#include <string.h>
+
+#include "../../include/pot_amd.h"
+#include "host.h"
+#include "internal.h"
+
+namespace potamd {
+namespace {
+
+constexpr int kFH = 256;             // hidden width
+constexpr int kFK = 32;              // input width (one K slice pair)
+constexpr int kFRows = 64;           // rows per block
+constexpr int kFStride = kFH + 8;    // bf16 per LDS row (16-byte pad against bank conflicts)
+constexpr int kFW = 4;               // waves per workgroup
+constexpr float kFLeak = 0.01f;
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+struct FusedMlpArgs {
+  int mode;
+  int rows, seg_len;
+  const uint16_t* inp;   // rows x 32 bf16
+  const uint16_t* W1;    // 32 x 256 bf16, row-major [k][j]
+  const float* bias1;    // mode 0: b1[256]; mode 1: rowvec[rows / seg_len][256]
+  const uint16_t* W2;    // 256 x 256 bf16, row-major [k][j]
+  const float* b2;       // [256]
+  const float* w3;       // mode 1: [256]
+  const float* b3;       // mode 1: [1]
+  const float* qn;       // mode 1: [rows] or null
+  float* out;
+};
+
+__device__ __forceinline__ float bf_to_f(unsigned h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ unsigned f_to_bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffff) ? 0x40u : 0u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+__device__ __forceinline__ float leaky(float x) { return x > 0.f ? x : kFLeak * x; }
+
+__device__ __forceinline__ f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// Eight bf16 of column j, rows k0 .. k0+7, of a row-major [k][256] matrix.
+__device__ __forceinline__ u32x4 load_col8(const uint16_t* W, int k0, int j) {
+  u32x4 v;
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    v[e] = (unsigned)W[(k0 + 2 * e) * kFH + j] | ((unsigned)W[(k0 + 2 * e + 1) * kFH + j] << 16);
+  return v;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kFW * kWave) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void mlp_fused_kernel(FusedMlpArgs A) {
+  __shared__ __attribute__((aligned(16))) uint16_t act[2][kFRows][kFStride];
+  __shared__ float part[2][kFW][kFRows];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int h = lane >> 5, l32 = lane & 31;
+  const int jbase = 64 * w;
+
+  // Weight slices: column j = jbase + 32 c + l32, rows 16 s + 8 h .. +7.
+  u32x4 w1r[2][2], w2r[2][16];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) w1r[c][s] = load_col8(A.W1, 16 * s + 8 * h, jbase + 32 * c + l32);
+#pragma unroll
+    for (int s = 0; s < 16; ++s) w2r[c][s] = load_col8(A.W2, 16 * s + 8 * h, jbase + 32 * c + l32);
+  }
+  // Per-lane constants: layer-2 bias (and head weight) of this lane's columns.
+  // In mode 0 a lane owns column jbase + 32 c + l32; in mode 1 the columns
+  // jbase + 32 c + 8 q + 4 h + i.
+  float b2c[2] = {A.b2[jbase + l32], A.b2[jbase + 32 + l32]};
+
+  const int nseg = A.rows / A.seg_len;
+  const int bps = A.seg_len / kFRows;
+  int buf = 0;
+  for (int p = blockIdx.x; p < nseg; p += gridDim.x) {
+    float zacc[2] = {0.f, 0.f};
+    const float* bias1 = MODE == 0 ? A.bias1 : A.bias1 + (int64_t)p * kFH;
+    for (int blk = 0; blk < bps; ++blk) {
+      const int64_t r0 = (int64_t)p * A.seg_len + (int64_t)blk * kFRows;
+      // ---- layer 1, transposed: D[j][r], r = 32 t + l32 ----
+      u32x4 xin[2][2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+          xin[t][s] = *reinterpret_cast<const u32x4*>(A.inp + (r0 + 32 * t + l32) * kFK + 16 * s +
+                                                      8 * h);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          f32x16 acc = {};
+          acc = mfma16(w1r[c][0], xin[t][0], acc);
+          acc = mfma16(w1r[c][1], xin[t][1], acc);
+          // lane value 4 q + i: column jbase + 32 c + 8 q + 4 h + i, row 32 t + l32
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j0 = jbase + 32 * c + 8 * q + 4 * h;
+            unsigned v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = f_to_bf(leaky(acc[4 * q + i] + bias1[j0 + i]));
+            u32x2 pk;
+            pk[0] = v[0] | (v[1] << 16);
+            pk[1] = v[2] | (v[3] << 16);
+            *reinterpret_cast<u32x2*>(&act[buf][32 * t + l32][j0]) = pk;
+          }
+        }
+      }
+      __syncthreads();
+      // ---- layer 2 ----
+      f32x16 acc[2][2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) acc[c][t] = f32x16{};
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        u32x4 a[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+          a[t] = *reinterpret_cast<const u32x4*>(&act[buf][32 * t + l32][16 * s + 8 * h]);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int t = 0; t < 2; ++t)
+            acc[c][t] = MODE == 0 ? mfma16(a[t], w2r[c][s], acc[c][t])    // D[r][j]
+                                  : mfma16(w2r[c][s], a[t], acc[c][t]);   // D[j][r]
+      }
+      if constexpr (MODE == 0) {
+        // lane value 4 q + i: row 32 t + 8 q + 4 h + i, column jbase + 32 c + l32
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+              zacc[c] += bf_to_f(f_to_bf(leaky(acc[c][t][v] + b2c[c])));
+      } else {
+        // lane value 4 q + i: column jbase + 32 c + 8 q + 4 h + i, row 32 t + l32
+        float fp[2] = {0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int j = jbase + 32 * c + 8 * q + 4 * h + i;
+              const float bj = A.b2[j], wj = A.w3[j];
+#pragma unroll
+              for (int t = 0; t < 2; ++t)
+                fp[t] = __builtin_fmaf(bf_to_f(f_to_bf(leaky(acc[c][t][4 * q + i] + bj))), wj,
+                                       fp[t]);
+            }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          fp[t] += __shfl_xor(fp[t], 32, kWave);
+          if (h == 0) part[buf][w][32 * t + l32] = fp[t];
+        }
+        __syncthreads();
+        if (tid < kFRows) {
+          float f = ((part[buf][0][tid] + part[buf][1][tid]) + part[buf][2][tid]) +
+                    part[buf][3][tid];
+          f += A.b3[0] + (A.qn ? A.qn[r0 + tid] : 0.f);
+          A.out[r0 + tid] = f;
+        }
+      }
+      buf ^= 1;
+    }
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float z = zacc[c] + __shfl_xor(zacc[c], 32, kWave);
+        if (h == 0) A.out[(int64_t)p * kFH + jbase + 32 * c + l32] = z / (float)A.seg_len;
+      }
+    }
+  }
+}
+
+}  // namespace
+}  // namespace potamd
+
+using namespace potamd;
+
+extern "C" int potamd_mlp_fused(int mode, int rows, int seg_len, const void* inp,
+                                const void* W1, const float* bias1, const void* W2,
+                                const float* b2, const float* w3, const float* b3,
+                                const float* qn, float* out, void* stream) {
+  if (rows < 1 || seg_len < 1 || !inp || !W1 || !bias1 || !W2 || !b2 || !out ||
+      (mode != 0 && mode != 1) || (mode == 1 && (!w3 || !b3)))
+    return POTAMD_EINVAL;
+  if (seg_len % kFRows != 0 || rows % seg_len != 0) return POTAMD_EUNSUPPORTED;
+  FusedMlpArgs fa;
+  memset(&fa, 0, sizeof(fa));
+  fa.mode = mode;
+  fa.rows = rows;
+  fa.seg_len = seg_len;
+  fa.inp = static_cast<const uint16_t*>(inp);
+  fa.W1 = static_cast<const uint16_t*>(W1);
+  fa.bias1 = bias1;
+  fa.W2 = static_cast<const uint16_t*>(W2);
+  fa.b2 = b2;
+  fa.w3 = w3;
+  fa.b3 = b3;
+  fa.qn = qn;
+  fa.out = out;
+  const int nseg = rows / seg_len;
+  const int grid = nseg < device_cus() ? nseg : device_cus();
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (mode == 0)
+    hipLaunchKernelGGL(mlp_fused_kernel<0>, dim3(grid), dim3(kFW * kWave), 0, s, fa);
+  else
+    hipLaunchKernelGGL(mlp_fused_kernel<1>, dim3(grid), dim3(kFW * kWave), 0, s, fa);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}

@@ -312,26 +312,28 @@ def _points_order(ns, ms, regs):
 
 
 def _launch_points(lib, P, X, Y, regs, u, v, niter, status, e_last, e_first, cmax, order,
-                   numItermax, check_every, stopThr, fallback_thr, stream, cmax_given=False):
+                   numItermax, check_every, stopThr, fallback_thr, stream, cmax_given=False,
+                   waves=0):
     mode = (2 if cmax_given else 1) if P.normalize else 0
     rc = lib.potamd_sinkhorn_log_batch_points(
         P.batch, int(P.ns.max()), int(P.ms.max()), P.d, _ptr(X), _ptr(Y), _ptr(P.aoff),
         _ptr(P.boff), _ptr(P.ns_d), _ptr(P.ms_d), _ptr(regs), mode, _ptr(P.a),
         _ptr(P.b), _ptr(u), _ptr(v), int(numItermax), int(check_every), float(stopThr),
         float(fallback_thr), _ptr(order), _ptr(niter), _ptr(status), _ptr(e_last),
-        _ptr(e_first), _ptr(cmax), stream)
+        _ptr(e_first), _ptr(cmax), int(waves), stream)
     _lib.check(rc, "potamd_sinkhorn_log_batch_points")
 
 
 def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=None,
-                               fallback_thr=0.0, check_every=10, cmax=None):
+                               fallback_thr=0.0, check_every=10, cmax=None, waves=0):
     """``sinkhorn_log_ragged`` on point clouds, the cost computed on the fly.
 
     P: a RaggedPoints.  Same loop, stop rule, warm start and fallback as
     ``sinkhorn_log_ragged``; the cost of problem p is |x - y|^2 (divided by
     its maximum when P.normalize), evaluated on MFMA from the points in every
     sweep and never stored.  ``cmax``: the maxima from ``points_cmax`` (the
-    solve then skips its own search).  Returns a RaggedResult whose ``cmax``
+    solve then skips its own search).  ``waves``: wavefronts per problem (8
+    or 16; 0 = the library default).  Returns a RaggedResult whose ``cmax``
     holds the cost maxima; asynchronous on the current stream.
     """
     lib = _lib.load()
@@ -354,7 +356,7 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
         order = torch.as_tensor(_check_order(order, B), device=dev)
     _launch_points(lib, P, P.X, P.Y, P.regs, u, v, niter, status, e_last, e_first, cmax, order,
                    numItermax, check_every, stopThr, fallback_thr,
-                   ctypes.c_void_p(_device.stream_ptr(dev)), cmax_given=given)
+                   ctypes.c_void_p(_device.stream_ptr(dev)), cmax_given=given, waves=waves)
     return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
                         err_last=e_last, cmax=cmax)
 

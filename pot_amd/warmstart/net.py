@@ -23,7 +23,8 @@ Every forward/backward step runs in libpot_amd.so: the four wide layers on
 the bf16 MFMA GEMM (fp32 accumulation, fused bias / per-problem broadcast /
 ReLU / ReLU-mask epilogues), the rest on small fp32 kernels; weights are
 fp32 masters updated by the library's Adam kernel, with bf16 copies for the
-GEMMs.  The float64 NumPy restatement in oracle/warmnet.py is the checker.
+GEMMs.  Inference runs each two-layer chain (and its segment mean or head)
+as one fused kernel whose hidden activations never reach HBM.  The float64 NumPy restatement in oracle/warmnet.py is the checker.
 """
 
 import ctypes
@@ -159,8 +160,39 @@ class WarmStartNet:
         self._op(OP_SEGMEAN, seg_len=seg, P=P, k=H, x=B, out=z)
         return A, B, z
 
+    def _fused(self, mode, inp, rows, seg, W1, bias1, W2, b2, out, w3=None, b3=None, qn=None):
+        rc = self.lib.potamd_mlp_fused(mode, rows, seg, _p(inp), _p(self.wb[W1]), _p(bias1),
+                                       _p(self.wb[W2]), _p(self.w[b2]), _p(w3), _p(b3), _p(qn),
+                                       _p(out), self._stream())
+        _lib.check(rc, "potamd_mlp_fused")
+
+    def forward_fused(self, Sin, Ein, qn, P, n, m):
+        """Inference forward with the fused two-layer kernel (n, m multiples of
+        64): the hidden activations stay on chip; same result as ``forward``
+        up to fp32 accumulation order."""
+        dev = self.device
+        w = self.w
+        zt = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._fused(0, Ein, P * m, m, "W_e1", w["b_e1"], "W_e2", "b_e2", zt)
+        zs = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._fused(0, Sin, P * n, n, "W_s1", w["b_s1"], "W_s2", "b_s2", zs)
+        c = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._op(OP_VECMAT, P=P, k=H, n=H, x=zt, w=w["W_m1z"], y=w["b_m1"], out=c)
+        c2 = torch.empty_like(c)
+        self._op(OP_VECMAT, P=P, k=H, n=H, flag=1, x=zs, w=w["W_m1s"], y=c, out=c2)
+        f = torch.empty(P * n, dtype=torch.float32, device=dev)
+        self._fused(1, Sin, P * n, n, "W_m1x", c2, "W_m2", "b_m2", f, w3=w["w_m3"],
+                    b3=w["b_m3"], qn=qn)
+        return f
+
     def forward(self, Sin, Ein, qn, P, n, m, keep=False):
-        """f (P*n,) from source rows Sin, target rows Ein and source norms qn."""
+        """f (P*n,) from source rows Sin, target rows Ein and source norms qn.
+
+        Without ``keep`` (inference) and with n, m multiples of 64 this is the
+        fused kernel; training keeps the per-layer GEMM path, whose
+        activations the backward pass reads."""
+        if not keep and n % 64 == 0 and m % 64 == 0:
+            return self.forward_fused(Sin, Ein, qn, P, n, m)
         dev = self.device
         w, wb = self.w, self.wb
         H1, H2, zt = self._encode(Ein, P * m, P, m, "W_e1", "b_e1", "W_e2", "b_e2")

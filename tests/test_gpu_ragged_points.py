@@ -29,13 +29,14 @@ def _plan(M, u, v, reg):
     return np.exp(-M / reg + u[:, None] + v[None, :])
 
 
-def test_points_fixed_iterations_match_oracle():
+@pytest.mark.parametrize("waves", [8, 16])
+def test_points_fixed_iterations_match_oracle(waves):
     # Uneven n, m (not multiples of 32), d < 16.
-    fam = problems.ragged_family(12, seed=321, n_range=(20, 300), d=5, kmin=1, kmax=3,
+    fam = problems.ragged_family(12, seed=321, n_range=(20, 600), d=5, kmin=1, kmax=3,
                                  reg_range=(0.02, 0.2))
     P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
     it = 60
-    res = ragged.sinkhorn_log_ragged_points(P, numItermax=it, stopThr=0.0)
+    res = ragged.sinkhorn_log_ragged_points(P, numItermax=it, stopThr=0.0, waves=waves)
     torch.cuda.synchronize()
     print("fixed-iteration batch done", flush=True)
     assert (res.niter.cpu().numpy() == it - 1).all()
@@ -106,12 +107,14 @@ def test_points_full_size_problem_matches_oracle():
     assert rel < 1e-5, rel
 
 
-def test_points_agree_with_materialised_batch():
+@pytest.mark.parametrize("waves", [8, 16])
+def test_points_agree_with_materialised_batch(waves):
     fam = problems.ragged_family(24, seed=11, n_range=(64, 1024), d=16, kmin=1, kmax=4,
                                  reg_range=(5e-3, 5e-2))
     P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
     R = ragged.from_clouds(fam.xs, fam.ys, fam.regs)
-    rp = ragged.sinkhorn_log_ragged_points(P, numItermax=2000, stopThr=1e-6, check_every=1)
+    rp = ragged.sinkhorn_log_ragged_points(P, numItermax=2000, stopThr=1e-6, check_every=1,
+                                           waves=waves)
     rd = ragged.sinkhorn_log_ragged(R, numItermax=2000, stopThr=1e-6, check_every=1)
     torch.cuda.synchronize()
     print("points vs materialised done", flush=True)

@@ -185,3 +185,44 @@ def test_training_lowers_loss():
     l1 = wtrain.evaluate(net, data, batch=8)
     print("loss before / after", l0, l1, flush=True)
     assert l1 < 0.6 * l0, (l0, l1)
+
+
+def test_fused_forward_matches_gemm_path_and_oracle():
+    # The inference kernel (two layers per launch, activations on chip) must
+    # give the per-layer GEMM path's result up to fp32 accumulation order, and
+    # the oracle's within the bf16 bar of the GEMM path.
+    dev = torch.device("cuda")
+    P, n, m = 2, 128, 192
+    params = wnet.WarmStartNet.init_params(4)
+    rng = np.random.default_rng(6)
+    for k in ("b_e1", "b_e2", "b_s1", "b_s2", "b_m1", "b_m2"):
+        params[k] = rng.standard_normal(params[k].shape) * 0.1
+    params["w_m3"] = rng.standard_normal(params["w_m3"].shape) * 0.05
+    params["b_m3"] = np.array([0.3])
+    net = wnet.WarmStartNet(params, dev)
+    X, Y, a, b, regs, cmax, _ = _net_inputs(net, P, n, m, 2)
+    Sin, Ein, qn = net.inputs(X, Y, a, b, regs, cmax, P, n, m)
+    f_gemm = net.forward(Sin, Ein, qn, P, n, m, keep=True)
+    f_fused = net.forward(Sin, Ein, qn, P, n, m, keep=False)
+    torch.cuda.synchronize()
+    err = _rel(f_fused.cpu().numpy(), f_gemm.cpu().numpy())
+    print("fused vs GEMM path:", err, flush=True)
+    assert err < 2e-3, err
+
+    w = net.params_numpy()
+    for k in wnet.GEMM_WEIGHTS:
+        w[k] = net.wb[k].float().cpu().double().numpy()
+    op = {k: w[k] for k in own.PARAMS if k not in ("w_m3",)}
+    for k in ("W_e1", "W_s1", "W_m1x"):
+        op[k] = w[k][:18]
+    op["w_m3"] = w["w_m3"][:, None]
+    S = Sin.float().cpu().double().numpy()
+    E = Ein.float().cpu().double().numpy()
+    xs = [S[p * n:(p + 1) * n, :16] for p in range(P)]
+    ys = [E[p * m:(p + 1) * m, :16] for p in range(P)]
+    as_ = [(S[p * n:(p + 1) * n, 16] + 1.0) / n for p in range(P)]
+    bs = [(E[p * m:(p + 1) * m, 16] + 1.0) / m for p in range(P)]
+    oregs = [float(np.exp(E[p * m, 17] * own.LR_S + own.LR_C)) for p in range(P)]
+    fo, _ = own.forward(op, xs, ys, as_, bs, oregs)
+    err_o = _rel(f_fused.cpu().numpy(), np.concatenate(fo))
+    assert err_o < 3e-2, err_o
