@@ -251,10 +251,15 @@ int potamd_mlp_op(int op, int rows, int seg_len, int P, int k, int n, int64_t co
  *   mode 0: out[s][:] = mean of A2 over the rows of segment s (bias1 = b1[256]);
  *   mode 1: out[r] = A2[r] . w3 + b3[0] + qn[r] (bias1 = rowvec[segment][256],
  *           qn may be NULL).
- * inp: rows x 32 bf16; W1: 32 x 256 and W2: 256 x 256 bf16, row-major.
+ * inp: rows x 32 bf16 input rows (16-byte aligned), or NULL to build them
+ *    from pts (rows x 16 fp32, 16-byte aligned), wts (per-row weight, fp64),
+ *    regs (fp64) and cmax (fp32) per segment as
+ *    [x * rsqrt(cmax), seg_len * w - 1, (log reg + 4.147366) / 1.151293, 0...].
+ * W1: 32 x 256 and W2: 256 x 256 bf16, row-major.
  * seg_len must be a multiple of 64 and divide rows.
  */
-int potamd_mlp_fused(int mode, int rows, int seg_len, const void* inp, const void* W1,
+int potamd_mlp_fused(int mode, int rows, int seg_len, const void* inp, const float* pts,
+                     const double* wts, const double* regs, const float* cmax, const void* W1,
                      const float* bias1, const void* W2, const float* b2, const float* w3,
                      const float* b3, const float* qn, float* out, void* stream);
 

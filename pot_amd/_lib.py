@@ -58,7 +58,8 @@ SIGNATURES = {
         c_int, c_int, c_int, c_vp, c_i64, c_int, c_vp, c_i64, c_int, c_vp, c_i64,
         c_int, ctypes.c_float, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_int, c_vp]),
     "potamd_mlp_fused": (c_int, [
-        c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+        c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,   # mode, rows, seg, inp, pts, wts, regs, cmax
+        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "potamd_mlp_op": (c_int, [
         c_int, c_int, c_int, c_int, c_int, c_int, c_i64, c_int, c_dbl, c_vp,
         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

@@ -149,6 +149,12 @@ struct GemmArgs {
   int64_t ldmask;
 };
 
+// Warm-start network input features: the weight column enters as
+// seg_len * w - 1 (0 for uniform weights) and log reg as
+// (log reg - log 0.0158) / 1.1513, i.e. [-1, 1] over reg in [5e-3, 5e-2].
+constexpr double kLogRegCenter = -4.147366;
+constexpr double kLogRegScale = 1.151293;
+
 enum MlpOpCode : int {
   kOpBuildInputs = 0,
   kOpSegmentMean = 1,

@@ -30,11 +30,6 @@ constexpr int kTM = 64, kTN = 64, kTK = 32, kPad = 8;
 // Activation of every hidden layer: leaky ReLU (no unit can die).  A stored
 // activation has the sign of its pre-activation, so backward masks read it.
 constexpr float kLeak = 0.01f;
-// Input features: the weight column enters as seg_len * w - 1 (0 for uniform
-// weights) and log reg as (log reg - log 0.0158) / 1.1513, i.e. [-1, 1] over
-// reg in [5e-3, 5e-2].
-constexpr double kLogRegCenter = -4.147366;
-constexpr double kLogRegScale = 1.151293;
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   __shared__ uint16_t As[kTM][kTK + kPad];

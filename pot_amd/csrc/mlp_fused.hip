@@ -14,6 +14,11 @@
 // GEMMs with the same epilogues (bias / broadcast, leaky ReLU, bf16 store),
 // a segment mean and a head kernel.
 //
+// The input rows are either given (bf16, rows x 32) or built here from the
+// fp32 points (rows x 16), the per-row weights and the per-segment reg and
+// cost maximum, with the arithmetic of the network's input-row kernel:
+// [x * rsqrt(cmax), seg_len * w - 1, (log reg - c) / s, 0 ...].
+//
 // Layout: a 256-thread workgroup (one wave per SIMD) walks whole segments;
 // wave w owns hidden columns 64 w .. 64 w + 63 of both layers and keeps its
 // slices of W1 (32 x 64) and W2 (256 x 64) in registers for the whole
@@ -23,7 +28,8 @@
 // mode 0 layer 2 gives D[r][j], so a lane owns a column and the segment sum
 // is a per-lane accumulation; in mode 1 it gives D[j][r] (operands swapped),
 // so a lane owns a row and the head's dot product is per lane.  A1 is double
-// buffered in LDS: one barrier per block (two in mode 1).
+// buffered in LDS: one barrier per block (two in mode 1).  The next block's
+// inputs are fetched while the current one is computed.
 #include <string.h>
 
 #include "../../include/pot_amd.h"
@@ -35,20 +41,27 @@ namespace {
 
 constexpr int kFH = 256;             // hidden width
 constexpr int kFK = 32;              // input width (one K slice pair)
+constexpr int kFD = 16;              // point coordinates
 constexpr int kFRows = 64;           // rows per block
 constexpr int kFStride = kFH + 8;    // bf16 per LDS row (16-byte pad against bank conflicts)
 constexpr int kFW = 4;               // waves per workgroup
 constexpr float kFLeak = 0.01f;
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 struct FusedMlpArgs {
   int mode;
   int rows, seg_len;
-  const uint16_t* inp;   // rows x 32 bf16
+  const uint16_t* inp;   // rows x 32 bf16, or null: built from the points
+  const float* pts;      // rows x 16 fp32
+  const double* wts;     // rows (a or b)
+  const double* regs;    // rows / seg_len
+  const float* cmax;     // rows / seg_len
   const uint16_t* W1;    // 32 x 256 bf16, row-major [k][j]
   const float* bias1;    // mode 0: b1[256]; mode 1: rowvec[rows / seg_len][256]
   const uint16_t* W2;    // 256 x 256 bf16, row-major [k][j]
@@ -60,12 +73,12 @@ struct FusedMlpArgs {
 };
 
 __device__ __forceinline__ float bf_to_f(unsigned h) { return __uint_as_float(h << 16); }
-__device__ __forceinline__ unsigned f_to_bf(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffff) ? 0x40u : 0u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
+// Two floats to packed bf16 (round to nearest even, v_cvt_pk_bf16_f32).
+__device__ __forceinline__ unsigned pack_bf2(float lo, float hi) {
+  const bf16x2 v = {(__bf16)lo, (__bf16)hi};
+  return __builtin_bit_cast(unsigned, v);
 }
+__device__ __forceinline__ float round_bf(float x) { return (float)(__bf16)x; }
 __device__ __forceinline__ float leaky(float x) { return x > 0.f ? x : kFLeak * x; }
 
 __device__ __forceinline__ f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c) {
@@ -82,7 +95,66 @@ __device__ __forceinline__ u32x4 load_col8(const uint16_t* W, int k0, int j) {
   return v;
 }
 
-template <int MODE>
+// The layer-1 B operands of one block: row r = r0 + 32 t + l32, input
+// features 16 s + 8 h .. +7.  Raw holds what the loads return (fetched one
+// block ahead); make() turns it into the bf16 operands.
+template <int IN>
+struct BlockIn;
+
+template <>
+struct BlockIn<0> {  // given bf16 rows
+  u32x4 raw[2][2];
+  __device__ void fetch(const FusedMlpArgs& A, int64_t r0, int h, int l32) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+        raw[t][s] = *reinterpret_cast<const u32x4*>(A.inp + (r0 + 32 * t + l32) * kFK + 16 * s +
+                                                    8 * h);
+  }
+  __device__ void make(const FusedMlpArgs&, int, int, u32x4 (&x)[2][2]) const {
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) x[t][s] = raw[t][s];
+  }
+};
+
+template <>
+struct BlockIn<1> {  // built from the points
+  f32x4 c[2][2];     // [t][half of the lane's 8 coordinates]
+  double w[2];
+  __device__ void fetch(const FusedMlpArgs& A, int64_t r0, int h, int l32) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int64_t r = r0 + 32 * t + l32;
+      const f32x4* p = reinterpret_cast<const f32x4*>(A.pts + r * kFD + 8 * h);
+      c[t][0] = p[0];
+      c[t][1] = p[1];
+      w[t] = A.wts[r];
+    }
+  }
+  // seg: the block's segment (scale and log reg are per segment).
+  __device__ void make(const FusedMlpArgs& A, int seg, int h, u32x4 (&x)[2][2]) const {
+    const float scale = rsqrtf(A.cmax[seg]);
+    const float lr = (float)((log(A.regs[seg]) - kLogRegCenter) / kLogRegScale);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        x[t][0][2 * q] = pack_bf2(c[t][q][0] * scale, c[t][q][1] * scale);
+        x[t][0][2 * q + 1] = pack_bf2(c[t][q][2] * scale, c[t][q][3] * scale);
+      }
+      const float wf = (float)(w[t] * A.seg_len - 1.0);
+      x[t][1][0] = h == 0 ? pack_bf2(wf, lr) : 0u;
+      x[t][1][1] = 0u;
+      x[t][1][2] = 0u;
+      x[t][1][3] = 0u;
+    }
+  }
+};
+
+template <int MODE, int IN>
 __global__ __launch_bounds__(kFW * kWave) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void mlp_fused_kernel(FusedMlpArgs A) {
   __shared__ __attribute__((aligned(16))) uint16_t act[2][kFRows][kFStride];
@@ -100,27 +172,31 @@ void mlp_fused_kernel(FusedMlpArgs A) {
 #pragma unroll
     for (int s = 0; s < 16; ++s) w2r[c][s] = load_col8(A.W2, 16 * s + 8 * h, jbase + 32 * c + l32);
   }
-  // Per-lane constants: layer-2 bias (and head weight) of this lane's columns.
-  // In mode 0 a lane owns column jbase + 32 c + l32; in mode 1 the columns
-  // jbase + 32 c + 8 q + 4 h + i.
-  float b2c[2] = {A.b2[jbase + l32], A.b2[jbase + 32 + l32]};
+  // In mode 0 a lane owns column jbase + 32 c + l32 of layer 2.
+  const float b2c[2] = {A.b2[jbase + l32], A.b2[jbase + 32 + l32]};
 
   const int nseg = A.rows / A.seg_len;
   const int bps = A.seg_len / kFRows;
+  BlockIn<IN> nxt;
+  if (blockIdx.x < nseg) nxt.fetch(A, (int64_t)blockIdx.x * A.seg_len, h, l32);
   int buf = 0;
   for (int p = blockIdx.x; p < nseg; p += gridDim.x) {
     float zacc[2] = {0.f, 0.f};
     const float* bias1 = MODE == 0 ? A.bias1 : A.bias1 + (int64_t)p * kFH;
     for (int blk = 0; blk < bps; ++blk) {
       const int64_t r0 = (int64_t)p * A.seg_len + (int64_t)blk * kFRows;
-      // ---- layer 1, transposed: D[j][r], r = 32 t + l32 ----
       u32x4 xin[2][2];
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-          xin[t][s] = *reinterpret_cast<const u32x4*>(A.inp + (r0 + 32 * t + l32) * kFK + 16 * s +
-                                                      8 * h);
+      nxt.make(A, p, h, xin);
+      // Fetch the next block (this segment's next, or the next segment's first).
+      {
+        int64_t rn = -1;
+        if (blk + 1 < bps)
+          rn = r0 + kFRows;
+        else if (p + (int)gridDim.x < nseg)
+          rn = (int64_t)(p + gridDim.x) * A.seg_len;
+        if (rn >= 0) nxt.fetch(A, rn, h, l32);
+      }
+      // ---- layer 1, transposed: D[j][r], r = 32 t + l32 ----
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
 #pragma unroll
@@ -132,12 +208,12 @@ void mlp_fused_kernel(FusedMlpArgs A) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int j0 = jbase + 32 * c + 8 * q + 4 * h;
-            unsigned v[4];
+            float v[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = f_to_bf(leaky(acc[4 * q + i] + bias1[j0 + i]));
+            for (int i = 0; i < 4; ++i) v[i] = leaky(acc[4 * q + i] + bias1[j0 + i]);
             u32x2 pk;
-            pk[0] = v[0] | (v[1] << 16);
-            pk[1] = v[2] | (v[3] << 16);
+            pk[0] = pack_bf2(v[0], v[1]);
+            pk[1] = pack_bf2(v[2], v[3]);
             *reinterpret_cast<u32x2*>(&act[buf][32 * t + l32][j0]) = pk;
           }
         }
@@ -169,8 +245,7 @@ void mlp_fused_kernel(FusedMlpArgs A) {
 #pragma unroll
           for (int t = 0; t < 2; ++t)
 #pragma unroll
-            for (int v = 0; v < 16; ++v)
-              zacc[c] += bf_to_f(f_to_bf(leaky(acc[c][t][v] + b2c[c])));
+            for (int v = 0; v < 16; ++v) zacc[c] += round_bf(leaky(acc[c][t][v] + b2c[c]));
       } else {
         // lane value 4 q + i: column jbase + 32 c + 8 q + 4 h + i, row 32 t + l32
         float fp[2] = {0.f, 0.f};
@@ -184,8 +259,7 @@ void mlp_fused_kernel(FusedMlpArgs A) {
               const float bj = A.b2[j], wj = A.w3[j];
 #pragma unroll
               for (int t = 0; t < 2; ++t)
-                fp[t] = __builtin_fmaf(bf_to_f(f_to_bf(leaky(acc[c][t][4 * q + i] + bj))), wj,
-                                       fp[t]);
+                fp[t] = __builtin_fmaf(round_bf(leaky(acc[c][t][4 * q + i] + bj)), wj, fp[t]);
             }
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
@@ -218,19 +292,27 @@ void mlp_fused_kernel(FusedMlpArgs A) {
 using namespace potamd;
 
 extern "C" int potamd_mlp_fused(int mode, int rows, int seg_len, const void* inp,
-                                const void* W1, const float* bias1, const void* W2,
-                                const float* b2, const float* w3, const float* b3,
-                                const float* qn, float* out, void* stream) {
-  if (rows < 1 || seg_len < 1 || !inp || !W1 || !bias1 || !W2 || !b2 || !out ||
-      (mode != 0 && mode != 1) || (mode == 1 && (!w3 || !b3)))
+                                const float* pts, const double* wts, const double* regs,
+                                const float* cmax, const void* W1, const float* bias1,
+                                const void* W2, const float* b2, const float* w3,
+                                const float* b3, const float* qn, float* out, void* stream) {
+  if (rows < 1 || seg_len < 1 || !W1 || !bias1 || !W2 || !b2 || !out ||
+      (mode != 0 && mode != 1) || (mode == 1 && (!w3 || !b3)) ||
+      (!inp && (!pts || !wts || !regs || !cmax)))
     return POTAMD_EINVAL;
   if (seg_len % kFRows != 0 || rows % seg_len != 0) return POTAMD_EUNSUPPORTED;
+  // 16-byte vector loads of the input rows / points.
+  if ((uintptr_t)(inp ? inp : (const void*)pts) % 16 != 0) return POTAMD_EINVAL;
   FusedMlpArgs fa;
   memset(&fa, 0, sizeof(fa));
   fa.mode = mode;
   fa.rows = rows;
   fa.seg_len = seg_len;
   fa.inp = static_cast<const uint16_t*>(inp);
+  fa.pts = pts;
+  fa.wts = wts;
+  fa.regs = regs;
+  fa.cmax = cmax;
   fa.W1 = static_cast<const uint16_t*>(W1);
   fa.bias1 = bias1;
   fa.W2 = static_cast<const uint16_t*>(W2);
@@ -242,10 +324,15 @@ extern "C" int potamd_mlp_fused(int mode, int rows, int seg_len, const void* inp
   const int nseg = rows / seg_len;
   const int grid = nseg < device_cus() ? nseg : device_cus();
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (mode == 0)
-    hipLaunchKernelGGL(mlp_fused_kernel<0>, dim3(grid), dim3(kFW * kWave), 0, s, fa);
+  const dim3 g(grid), b(kFW * kWave);
+  if (mode == 0 && inp)
+    hipLaunchKernelGGL((mlp_fused_kernel<0, 0>), g, b, 0, s, fa);
+  else if (mode == 0)
+    hipLaunchKernelGGL((mlp_fused_kernel<0, 1>), g, b, 0, s, fa);
+  else if (inp)
+    hipLaunchKernelGGL((mlp_fused_kernel<1, 0>), g, b, 0, s, fa);
   else
-    hipLaunchKernelGGL(mlp_fused_kernel<1>, dim3(grid), dim3(kFW * kWave), 0, s, fa);
+    hipLaunchKernelGGL((mlp_fused_kernel<1, 1>), g, b, 0, s, fa);
   HIP_TRY(hipGetLastError());
   return 0;
 }

@@ -160,11 +160,37 @@ class WarmStartNet:
         self._op(OP_SEGMEAN, seg_len=seg, P=P, k=H, x=B, out=z)
         return A, B, z
 
-    def _fused(self, mode, inp, rows, seg, W1, bias1, W2, b2, out, w3=None, b3=None, qn=None):
-        rc = self.lib.potamd_mlp_fused(mode, rows, seg, _p(inp), _p(self.wb[W1]), _p(bias1),
-                                       _p(self.wb[W2]), _p(self.w[b2]), _p(w3), _p(b3), _p(qn),
-                                       _p(out), self._stream())
+    def _fused(self, mode, inp, rows, seg, W1, bias1, W2, b2, out, w3=None, b3=None, qn=None,
+               pts=None, wts=None, regs=None, cmax=None):
+        rc = self.lib.potamd_mlp_fused(mode, rows, seg, _p(inp), _p(pts), _p(wts), _p(regs),
+                                       _p(cmax), _p(self.wb[W1]), _p(bias1), _p(self.wb[W2]),
+                                       _p(self.w[b2]), _p(w3), _p(b3), _p(qn), _p(out),
+                                       self._stream())
         _lib.check(rc, "potamd_mlp_fused")
+
+    def forward_points(self, X, Y, a, b, regs, cmax, P, n, m):
+        """Inference forward straight from the points (n, m multiples of 64): the
+        fused kernels build the input rows themselves.  X: (P*n, 16) fp32,
+        a: (P*n,) fp64, regs: (P,) fp64, cmax: (P,) fp32."""
+        dev = self.device
+        w = self.w
+        X = X.reshape(-1, DX).float().contiguous()
+        Y = Y.reshape(-1, DX).float().contiguous()
+        zt = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._fused(0, None, P * m, m, "W_e1", w["b_e1"], "W_e2", "b_e2", zt, pts=Y, wts=b,
+                    regs=regs, cmax=cmax)
+        zs = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._fused(0, None, P * n, n, "W_s1", w["b_s1"], "W_s2", "b_s2", zs, pts=X, wts=a,
+                    regs=regs, cmax=cmax)
+        c = torch.empty((P, H), dtype=torch.float32, device=dev)
+        self._op(OP_VECMAT, P=P, k=H, n=H, x=zt, w=w["W_m1z"], y=w["b_m1"], out=c)
+        c2 = torch.empty_like(c)
+        self._op(OP_VECMAT, P=P, k=H, n=H, flag=1, x=zs, w=w["W_m1s"], y=c, out=c2)
+        qn = self.row_norms(X, P, n, cmax)
+        f = torch.empty(P * n, dtype=torch.float32, device=dev)
+        self._fused(1, None, P * n, n, "W_m1x", c2, "W_m2", "b_m2", f, w3=w["w_m3"],
+                    b3=w["b_m3"], qn=qn, pts=X, wts=a, regs=regs, cmax=cmax)
+        return f
 
     def forward_fused(self, Sin, Ein, qn, P, n, m):
         """Inference forward with the fused two-layer kernel (n, m multiples of
@@ -290,7 +316,10 @@ class WarmStartNet:
             raise ValueError("the warm-start network expects problems of one size")
         regs_t = torch.as_tensor(np.asarray(regs, dtype=np.float64)).to(self.device)
         cmax = R.cmax if R.cmax.dtype == torch.float32 else R.cmax.float()
-        Sin, Ein, qn = self.inputs(X.reshape(-1, DX).float(), Y.reshape(-1, DX).float(), R.a,
-                                   R.b, regs_t, cmax, P, n, m)
-        f = self.forward(Sin, Ein, qn, P, n, m)
+        if n % 64 == 0 and m % 64 == 0:
+            f = self.forward_points(X, Y, R.a, R.b, regs_t, cmax.contiguous(), P, n, m)
+        else:
+            Sin, Ein, qn = self.inputs(X.reshape(-1, DX).float(), Y.reshape(-1, DX).float(),
+                                       R.a, R.b, regs_t, cmax, P, n, m)
+            f = self.forward(Sin, Ein, qn, P, n, m)
         return (f.double().view(P, n) / regs_t[:, None]).reshape(-1)

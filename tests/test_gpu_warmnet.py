@@ -204,10 +204,14 @@ def test_fused_forward_matches_gemm_path_and_oracle():
     Sin, Ein, qn = net.inputs(X, Y, a, b, regs, cmax, P, n, m)
     f_gemm = net.forward(Sin, Ein, qn, P, n, m, keep=True)
     f_fused = net.forward(Sin, Ein, qn, P, n, m, keep=False)
+    # inputs built inside the fused kernels from the points
+    f_pts = net.forward_points(X, Y, a, b, regs, cmax, P, n, m)
     torch.cuda.synchronize()
     err = _rel(f_fused.cpu().numpy(), f_gemm.cpu().numpy())
-    print("fused vs GEMM path:", err, flush=True)
+    err_p = _rel(f_pts.cpu().numpy(), f_gemm.cpu().numpy())
+    print("fused vs GEMM path:", err, "from points:", err_p, flush=True)
     assert err < 2e-3, err
+    assert err_p < 2e-3, err_p
 
     w = net.params_numpy()
     for k in wnet.GEMM_WEIGHTS:
