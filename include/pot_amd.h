@@ -193,6 +193,12 @@ int potamd_sinkhorn_log_batch(int dtype, int batch, int max_n, int max_m, const 
  * cmax: per-problem cost maximum (device double[B]): written with normalize
  *    0 (1.0) or 1, read with 2; may be NULL unless normalize == 2.
  * waves: wavefronts per problem, 8 or 16 (0 = default 8).
+ * queue: device int[2], zero before the first launch; with it each
+ *    workgroup takes the next problem of `order` when it starts (global
+ *    longest-first scheduling whatever the block-to-CU assignment), and the
+ *    kernel leaves it zero again, so a buffer can be reused by later calls
+ *    and graph replays.  Calls that overlap on one GPU need separate queues.
+ *    NULL: workgroup i solves problem order[i].
  */
 int potamd_sinkhorn_log_batch_points(int batch, int max_n, int max_m, int d, const float* X,
                                      const float* Y, const int64_t* xoff, const int64_t* yoff,
@@ -201,7 +207,7 @@ int potamd_sinkhorn_log_batch_points(int batch, int max_n, int max_m, int d, con
                                      double* v, int max_iter, int check_every, double stop_thr,
                                      double fallback_thr, const int* order, int* niter,
                                      int* status, double* err_last, double* err_first,
-                                     double* cmax, int waves, void* stream);
+                                     double* cmax, int waves, int* queue, void* stream);
 
 /* Per-problem maximum of |x_r - y_c|^2 (the M.max() of ot.dist(x, y) /
  * M.max()), the same device search as potamd_sinkhorn_log_batch_points with

@@ -48,7 +48,7 @@ SIGNATURES = {
         c_vp, c_vp, c_vp, c_vp,                  # a, b, u, v
         c_int, c_int, c_dbl, c_dbl, c_vp,        # max_iter, check_every, stop, fallback, order
         c_vp, c_vp, c_vp, c_vp, c_vp,            # niter, status, err_last, err_first, cmax
-        c_int, c_vp]),                           # waves, stream
+        c_int, c_vp, c_vp]),                     # waves, queue, stream
     "potamd_ragged_points_cmax": (c_int, [
         c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "potamd_sqeuclidean_ragged": (c_int, [

@@ -83,6 +83,8 @@ struct PointsArgs {
   double* err_last;
   double* err_first;
   double* cmax;          // [B] the cost maximum used (out; in when normalize == 2); may be null
+  int batch;             // B
+  int* queue;            // [2] work-queue counters (zero at launch, zero at exit) or null
 };
 
 __device__ __forceinline__ unsigned bf16_bits(float f) {
@@ -156,6 +158,20 @@ __device__ __forceinline__ void point_ops(const float* p, bool valid, int d, con
     hi[k] = hb[2 * k] | (hb[2 * k + 1] << 16);
     lo[k] = lb[2 * k] | (lb[2 * k + 1] << 16);
   }
+}
+
+// Values that every lane of the workgroup holds (read back from LDS after a
+// barrier), made visibly uniform to the compiler so that the control flow
+// they steer -- around barriers -- is uniform.
+__device__ __forceinline__ int uniform_i(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uniform_f(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uniform_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 __device__ __forceinline__ f32x16 mfma_h(u32x4 a, u32x4 b, f32x16 c) {
@@ -332,7 +348,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   };
 
   // ---- cost maximum ----
-  double cmax = A.normalize == 2 ? A.cmax[p] : 1.0;
+  double cmax = A.normalize == 2 ? uniform_d(A.cmax[p]) : 1.0;
   if (A.normalize == 1) {
     const double sc0 = 1.4142135623730951 * (ext > 0.0 ? ldexp(1.0, -ilogb(ext) - 1) : 1.0);
     pack(sc0);
@@ -372,13 +388,14 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     __syncthreads();
     cmax = 0.0;
     for (int w = 0; w < PW; ++w) cmax = fmax(cmax, L.red[w]);
+    cmax = uniform_d(cmax);
     __syncthreads();
     if (A.max_only) {
       if (tid == 0) A.cmax[p] = cmax;
       return;
     }
   }
-  const double reg = A.regs[p] * cmax;
+  const double reg = uniform_d(A.regs[p]) * cmax;
   const double s = kLog2e / reg;
   {
     const float wm = wave_max(pack(sqrt(2.0 * s)));
@@ -387,6 +404,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   __syncthreads();
   float allmax = 0.f;
   for (int w = 0; w < PW; ++w) allmax = fmaxf(allmax, L.redf[w]);
+  allmax = uniform_f(allmax);
   if (!(allmax <= kMaxOperand) || !(cmax > 0.0) || !(reg > 0.0)) {
     // Outside the operand range of the f16 MFMA path (reg far below the
     // cost scale) or a degenerate cost: report instead of solving.
@@ -405,7 +423,10 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   const bool warm = A.fallback_thr > 0.0;
   bool cold = false;
 
-restart:
+  // One attempt from the given start; a rejected warm start repeats the
+  // prologue and the iterations once from the cold start.
+  for (;;) {
+  bool again = false;
   // ---- prologue: v_1 = log2 b - LSE2_r(x_rc + u2_r), a max pass then a sum pass ----
   for (int r = tid; r < nblk * 32; r += PT) {
     const double u2 = (r < n && !cold) ? up[r] * kLog2e : 0.0;
@@ -567,13 +588,14 @@ restart:
       L.flag = stop;
     }
     __syncthreads();
-    const int fl = L.flag;
+    const int fl = uniform_i(L.flag);
     if (fl == 2) {
       // Warm start rejected: restart from the cold start u0 = 0.
       restarted = 1;
       cold = true;
+      again = true;
       __syncthreads();
-      goto restart;
+      break;
     }
     if (fl == 1) {
       status = kConverged;
@@ -597,6 +619,8 @@ restart:
     }
     __syncthreads();
   }
+  if (!again) break;
+  }
 
   // ---- outputs (natural-log potentials) ----
   __syncthreads();
@@ -612,14 +636,24 @@ restart:
   }
 }
 
+// One problem, from the centring to the outputs (every thread of the
+// workgroup calls it; returns are uniform).  Kept out of line, with its LDS
+// declared here, so that the persistent kernel's loop around it is a plain
+// uniform loop.
 template <int PW>
-__global__ __launch_bounds__(PW * kWave) void batch_points_kernel(PointsArgs A) {
+__device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int p) {
+  __shared__ PointsLds<PW> L;
+  // Arguments of an out-of-line function count as per-lane values for the
+  // compiler: the ones that steer barriers are made visibly uniform.
+  PointsArgs A = Ain;
+  A.normalize = uniform_i(Ain.normalize);
+  A.max_only = uniform_i(Ain.max_only);
+  A.max_iter = uniform_i(Ain.max_iter);
+  p = uniform_i(p);
   constexpr int PT = PW * kWave;
   constexpr int NG = PT / 16;  // point groups of the centring reduction
-  __shared__ PointsLds<PW> L;
-  const int p = A.order ? A.order[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
-  const int n = A.ns[p], m = A.ms[p], d = A.d;
+  const int n = uniform_i(A.ns[p]), m = uniform_i(A.ms[p]), d = A.d;
   if (n < 1 || n > kPMax || m < 1 || m > kPMax) {
     if (tid == 0) {
       if (A.max_only) {
@@ -698,6 +732,37 @@ __global__ __launch_bounds__(PW * kWave) void batch_points_kernel(PointsArgs A) 
   }
 }
 
+// With a queue the launch is persistent, one workgroup per CU: each takes
+// the next problem of `order` from a device counter until the batch is
+// exhausted, so the CUs never wait for the dispatch of a new workgroup and
+// the longest-first order is kept across the whole GPU.  The last workgroup
+// to leave resets the counters to zero, which leaves the queue ready for
+// the next launch (also inside a graph).  Without a queue, workgroup i
+// solves problem order[i].
+template <int PW>
+__global__ __launch_bounds__(PW * kWave) void batch_points_kernel(PointsArgs A) {
+  __shared__ int qslot;
+  if (!A.queue) {
+    one_problem<PW>(A, A.order ? uniform_i(A.order[blockIdx.x]) : (int)blockIdx.x);
+    return;
+  }
+  for (;;) {
+    if (threadIdx.x == 0) qslot = atomicAdd(&A.queue[0], 1);
+    __syncthreads();
+    const int q = uniform_i(qslot);
+    __syncthreads();  // everyone has the slot before it is taken again
+    if (q < 0 || q >= A.batch) break;
+    one_problem<PW>(A, A.order ? uniform_i(A.order[q]) : q);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&A.queue[1], 1) == (int)gridDim.x - 1) {
+      atomicExch(&A.queue[0], 0);
+      atomicExch(&A.queue[1], 0);
+    }
+  }
+}
+
 }  // namespace
 }  // namespace potamd
 
@@ -708,12 +773,13 @@ extern "C" int potamd_sinkhorn_log_batch_points(
     const int64_t* yoff, const int* ns, const int* ms, const double* regs, int normalize,
     const double* a, const double* b, double* u, double* v, int max_iter, int check_every,
     double stop_thr, double fallback_thr, const int* order, int* niter, int* status,
-    double* err_last, double* err_first, double* cmax, int waves, void* stream) {
+    double* err_last, double* err_first, double* cmax, int waves, int* queue, void* stream) {
   if (batch < 1 || max_n < 1 || max_m < 1 || d < 1 || !X || !Y || !xoff || !yoff || !ns ||
       !ms || !regs || !a || !b || !u || !v || max_iter < 1 || !niter || !status ||
       check_every < 0 || normalize < 0 || normalize > 2 || (normalize == 2 && !cmax))
     return POTAMD_EINVAL;
-  if (max_n > kPMax || max_m > kPMax || d > kPD) return POTAMD_EUNSUPPORTED;
+  if (max_n > kPMax || max_m > kPMax || d > kPD || (queue && batch > (1 << 30)))
+    return POTAMD_EUNSUPPORTED;
   if (waves == 0) waves = kPWDefault;
   if (waves != 8 && waves != 16) return POTAMD_EINVAL;
   PointsArgs pa;
@@ -741,11 +807,16 @@ extern "C" int potamd_sinkhorn_log_batch_points(
   pa.err_last = err_last;
   pa.err_first = err_first;
   pa.cmax = cmax;
+  pa.batch = batch;
+  pa.queue = queue;
+  // Persistent launch with the queue: one workgroup per CU (the LDS of one
+  // problem fills most of a CU).
+  const int grid = queue ? (batch < device_cus() ? batch : device_cus()) : batch;
   if (waves == 16)
-    hipLaunchKernelGGL(batch_points_kernel<16>, dim3(batch), dim3(16 * kWave), 0,
+    hipLaunchKernelGGL(batch_points_kernel<16>, dim3(grid), dim3(16 * kWave), 0,
                        static_cast<hipStream_t>(stream), pa);
   else
-    hipLaunchKernelGGL(batch_points_kernel<8>, dim3(batch), dim3(8 * kWave), 0,
+    hipLaunchKernelGGL(batch_points_kernel<8>, dim3(grid), dim3(8 * kWave), 0,
                        static_cast<hipStream_t>(stream), pa);
   HIP_TRY(hipGetLastError());
   return 0;
@@ -771,6 +842,7 @@ extern "C" int potamd_ragged_points_cmax(int batch, int max_n, int max_m, int d,
   pa.normalize = 1;
   pa.max_only = 1;
   pa.cmax = cmax;
+  pa.batch = batch;
   hipLaunchKernelGGL(batch_points_kernel<kPWDefault>, dim3(batch), dim3(kPWDefault * kWave), 0,
                      static_cast<hipStream_t>(stream), pa);
   HIP_TRY(hipGetLastError());

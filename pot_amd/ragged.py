@@ -313,14 +313,14 @@ def _points_order(ns, ms, regs):
 
 def _launch_points(lib, P, X, Y, regs, u, v, niter, status, e_last, e_first, cmax, order,
                    numItermax, check_every, stopThr, fallback_thr, stream, cmax_given=False,
-                   waves=0):
+                   waves=0, queue=None):
     mode = (2 if cmax_given else 1) if P.normalize else 0
     rc = lib.potamd_sinkhorn_log_batch_points(
         P.batch, int(P.ns.max()), int(P.ms.max()), P.d, _ptr(X), _ptr(Y), _ptr(P.aoff),
         _ptr(P.boff), _ptr(P.ns_d), _ptr(P.ms_d), _ptr(regs), mode, _ptr(P.a),
         _ptr(P.b), _ptr(u), _ptr(v), int(numItermax), int(check_every), float(stopThr),
         float(fallback_thr), _ptr(order), _ptr(niter), _ptr(status), _ptr(e_last),
-        _ptr(e_first), _ptr(cmax), int(waves), stream)
+        _ptr(e_first), _ptr(cmax), int(waves), _ptr(queue), stream)
     _lib.check(rc, "potamd_sinkhorn_log_batch_points")
 
 
@@ -354,9 +354,12 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
         order = torch.from_numpy(_points_order(P.ns, P.ms, regs_h)).to(dev)
     else:
         order = torch.as_tensor(_check_order(order, B), device=dev)
+    # work-queue counters of the persistent launch (the kernel leaves them zero)
+    queue = torch.zeros(2, dtype=torch.int32, device=dev)
     _launch_points(lib, P, P.X, P.Y, P.regs, u, v, niter, status, e_last, e_first, cmax, order,
                    numItermax, check_every, stopThr, fallback_thr,
-                   ctypes.c_void_p(_device.stream_ptr(dev)), cmax_given=given, waves=waves)
+                   ctypes.c_void_p(_device.stream_ptr(dev)), cmax_given=given, waves=waves,
+                   queue=queue)
     return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
                         err_last=e_last, cmax=cmax)
 
@@ -410,6 +413,7 @@ class RaggedGraph:
         self.e_first = torch.zeros(B, dtype=torch.float64, device=dev)
         self.e_last = torch.zeros(B, dtype=torch.float64, device=dev)
         self.order = torch.arange(B, dtype=torch.int32, device=dev)
+        self.queue = torch.zeros(2, dtype=torch.int32, device=dev)  # left zero by each launch
         self.settings = (int(numItermax), int(check_every), float(stopThr), float(fallback_thr),
                          int(waves) | (int(prefetch) << 8) | (int(occupancy) << 16))
         # eager warm-up on a side stream, then capture
@@ -436,7 +440,8 @@ class RaggedGraph:
             max_iter, check_every, stop, fb, _ = self.settings
             _launch_points(self.lib, self, self.X, self.Y, self.reg_in, self.u, self.v,
                            self.niter, self.status, self.e_last, self.e_first, self.cmax,
-                           self.order, max_iter, check_every, stop, fb, stream)
+                           self.order, max_iter, check_every, stop, fb, stream,
+                           queue=self.queue)
             return
         rc = self.lib.potamd_sqeuclidean_ragged(
             len(self.ns), int(self.ns.max()), int(self.ms.max()), self.d, _ptr(self.X),
