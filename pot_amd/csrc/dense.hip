@@ -127,45 +127,73 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
   const int nloc = g < n ? (n - g + G - 1) / G : 0;
   auto row_of = [&](int k) { return g + k * G; };
 
-  // Rows in flight: the current one plus PF prefetched (rotating registers).
-  R cur[CPT], nxt[CPT], nx2[PF > 1 ? CPT : 1];
+  // The per-row scalars of this workgroup's rows are staged in LDS before the
+  // row loop: a global load of them inside the loop would be issued after the
+  // row prefetch, and its wait (the vector memory counter drains in order)
+  // would also wait for the prefetch.
+  __shared__ double row_a[kMaxRowsPerBlock];
+  __shared__ double row_la[kMaxRowsPerBlock];
+  __shared__ double row_uo[kMaxRowsPerBlock];
+  for (int r = tid; r < nloc; r += T) {
+    const int i = row_of(r);
+    row_a[r] = ap[i];
+    row_la[r] = DOMAIN == kLog ? la2[i] : 0.0;
+    row_uo[r] = uo[i];
+  }
+  __syncthreads();
+
+  // PF + 1 row buffers: while row k is processed, rows k+1 .. k+PF are in
+  // flight.  The loop is unrolled over the buffers so they rotate without
+  // register copies (a copy would wait for the prefetch).  Every load is
+  // unconditional, because the compiler's wait counts assume the fewest
+  // loads any path may have issued: a lane past the last column loads
+  // column 0 of the row (its value is replaced by 0 where it is used), and a
+  // prefetch past the workgroup's last row reads that row again.
+  constexpr int NB = PF + 1;
+  const bool padded = m < T * CPT;
+  R buf[NB][CPT];
   auto load_row = [&](int i, R* dst) {
     const TM* row = Mp + (int64_t)i * A.ldm;
 #pragma unroll
     for (int k = 0; k < NCH; ++k) {
       const int j0 = (k * T + tid) * VEC;
       TM tmp[VEC];
-      if (j0 < m) {
-        load_vec<TM, VEC>(row + j0, tmp);
-      } else {
-#pragma unroll
-        for (int c = 0; c < VEC; ++c) tmp[c] = 0;
-      }
+      load_vec<TM, VEC>(row + (j0 < m ? j0 : 0), tmp);
 #pragma unroll
       for (int c = 0; c < VEC; ++c) dst[k * VEC + c] = (R)tmp[c];
     }
   };
-  if (nloc > 0) load_row(row_of(0), cur);
-  if constexpr (PF > 1) {
-    if (nloc > 1) load_row(row_of(1), nxt);
+  auto mask_row = [&](R* cur) {
+    if (padded) {
+#pragma unroll
+      for (int k = 0; k < NCH; ++k) {
+        const bool ok = (k * T + tid) * VEC < m;
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) cur[k * VEC + c] = ok ? cur[k * VEC + c] : (R)0;
+      }
+    }
+  };
+  if (nloc > 0) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      load_row(row_of(b < nloc ? b : nloc - 1), buf[b]);
+      // Issue the buffers in order: the waits inside the loop count on it.
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
 
   const R s_hi = (R)(sizeof(TM) == 4 ? (double)A.s_hi : A.s2);
   const R s_lo = (R)(sizeof(TM) == 4 ? (double)A.s_lo : 0.0);
 
-  for (int k = 0; k < nloc; ++k) {
+  auto process = [&](int k, R* cur) {
     const int i = row_of(k);
-    if constexpr (PF > 1) {
-      if (k + 2 < nloc) load_row(row_of(k + 2), nx2);
-    } else {
-      if (k + 1 < nloc) load_row(row_of(k + 1), nxt);
-    }
-    const double a_i = ap[i];
+    mask_row(cur);
+    const double a_i = row_a[k];
     if (DOMAIN == kLog) {
       if (a_i == 0.0) {
         if (tid == 0) { row_c[k] = __builtin_nan(""); row_s[k] = 0.0; }
       } else {
-        double c2 = la2[i] - uo[i];
+        double c2 = row_la[k] - row_uo[k];
         if (!__builtin_isfinite(c2)) c2 = 0.0;
         R s = 0;
 #pragma unroll
@@ -178,6 +206,7 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
         if (!in_safe_range(S)) {
           // Exact path: reload the row and shift by its maximum.
           load_row(i, cur);
+          mask_row(cur);
           R mx = -(R)__builtin_inf();
 #pragma unroll
           for (int q = 0; q < CPT; ++q) {
@@ -209,7 +238,7 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
       for (int q = 0; q < CPT; ++q)
         if (!A.kernel_given) cur[q] = exp2_fast(scaled_exponent(cur[q], s_hi, s_lo, (R)0));
       if (MODE == 1) {
-        const R f = (R)uo[i];
+        const R f = (R)row_uo[k];
 #pragma unroll
         for (int q = 0; q < CPT; ++q) acc[q] = __builtin_fma(cur[q], f, acc[q]);
       } else {
@@ -227,11 +256,14 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
         for (int q = 0; q < CPT; ++q) acc[q] = __builtin_fma(cur[q], f, acc[q]);
       }
     }
+  };
+
+  for (int k0 = 0; k0 < nloc; k0 += NB) {
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) cur[q] = nxt[q];
-    if constexpr (PF > 1) {
-#pragma unroll
-      for (int q = 0; q < CPT; ++q) nxt[q] = nx2[q];
+    for (int b = 0; b < NB; ++b) {
+      const int k = k0 + b;
+      if (k < nloc) process(k, buf[b]);
+      load_row(row_of(k + NB < nloc ? k + NB : nloc - 1), buf[b]);
     }
   }
 
@@ -291,24 +323,22 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
   }
 }
 
-// Column merge + marginal check + column update.  The last workgroup to
-// finish (ticket counter) reduces the per-block error partials in a fixed
-// order and takes the stop decision, so results are deterministic.
-// Column merge: a block owns kMergeCols columns; its 256 threads split the G
-// row-block partials of each column into 256 / kMergeCols groups, summed in
-// LDS (fixed order, so the result is deterministic).
+// Column merge + marginal check partials + column update; the stop decision
+// follows in col_decide_kernel.  A block owns kMergeCols columns; its 256 threads are 16
+// column quads (4 adjacent columns, one 16-byte load) by 16 groups of row
+// blocks, so each thread keeps all its loads in flight; the 16 group sums
+// meet in LDS and are added in a fixed order (deterministic result).
 constexpr int kMergeCols = 64;
 
 template <typename TP>
 __global__ __launch_bounds__(256) void col_merge_kernel(MergeArgs A) {
   Ctrl* ctrl = A.ctrl;
   if (ctrl->done) return;
-  constexpr int NG = 256 / kMergeCols;
+  constexpr int NQ = kMergeCols / 4;  // column quads per block
+  constexpr int NG = 256 / NQ;        // groups of row blocks
   __shared__ double gsum[NG][kMergeCols];
   const int p = blockIdx.y;
   const int tid = threadIdx.x;
-  const int cl = tid % kMergeCols, grp = tid / kMergeCols;
-  const int j = blockIdx.x * kMergeCols + cl;
   const int m = A.m;
   const bool col_first = A.order == kColFirst;
   const bool check_col = !A.prologue && col_first && (A.iter % 10 == 0);
@@ -316,21 +346,45 @@ __global__ __launch_bounds__(256) void col_merge_kernel(MergeArgs A) {
   const int next_slot = (A.iter + 1) & 1;
 
   {
-    double part = 0.0;
-    if (j < m) {
-      const TP* cp = static_cast<const TP*>(A.colpart) + (int64_t)p * A.G * m + j;
-      const int g0 = grp * ((A.G + NG - 1) / NG);
-      const int g1 = min(A.G, g0 + (A.G + NG - 1) / NG);
+    const int qd = tid % NQ, grp = tid / NQ;
+    const int j0 = blockIdx.x * kMergeCols + qd * 4;
+    const int per = (A.G + NG - 1) / NG;
+    const int g0 = grp * per;
+    const int g1 = min(A.G, g0 + per);
+    const TP* cp = static_cast<const TP*>(A.colpart) + (int64_t)p * A.G * m;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    bool vec = false;
+    if constexpr (sizeof(TP) == 4)
+      vec = (m & 3) == 0 && (reinterpret_cast<uintptr_t>(A.colpart) & 15) == 0;
+    if (vec) {
+      // m % 4 == 0 and j0 % 4 == 0: the quad is whole when it starts inside.
+      if (j0 < m) {
 #pragma unroll 8
-      for (int g = g0; g < g1; ++g) part += (double)cp[(int64_t)g * m];
+        for (int g = g0; g < g1; ++g) {
+          const f32x4 x = *reinterpret_cast<const f32x4*>(cp + (int64_t)g * m + j0);
+          part[0] += (double)x[0];
+          part[1] += (double)x[1];
+          part[2] += (double)x[2];
+          part[3] += (double)x[3];
+        }
+      }
+    } else {
+      for (int g = g0; g < g1; ++g) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (j0 + c < m) part[c] += (double)cp[(int64_t)g * m + j0 + c];
+      }
     }
-    gsum[grp][cl] = part;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) gsum[grp][qd * 4 + c] = part[c];
   }
   __syncthreads();
 
+  const int cl = tid;  // column of this block handled by threads 0..kMergeCols-1
+  const int j = blockIdx.x * kMergeCols + cl;
   double e2 = 0.0;
   int bad = 0, need_exact = 0;
-  if (grp == 0 && j < m) {
+  if (tid < kMergeCols && j < m) {
     double cs = 0.0;
 #pragma unroll
     for (int g = 0; g < NG; ++g) cs += gsum[g][cl];
@@ -363,26 +417,48 @@ __global__ __launch_bounds__(256) void col_merge_kernel(MergeArgs A) {
   if ((tid & 63) == 0) red[tid >> 6] = s;
   const int any_bad = __syncthreads_or(bad);
   const int any_exact = __syncthreads_or(need_exact);
-  const int nblk = gridDim.x;
   if (tid == 0) {
-    A.errpart[(int64_t)p * nblk + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    A.errpart[(int64_t)p * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
     if (any_bad) A.bad[next_slot] = 1;
     if (any_exact) atomicOr(&ctrl->need_exact, 1);
   }
-  __threadfence();
-  __shared__ int last;
-  if (tid == 0) {
-    const unsigned total = gridDim.x * gridDim.y;
-    const unsigned ticket = atomicAdd(&ctrl->counter, 1u);
-    last = (ticket == total - 1);
+}
+
+// Stop decision of one iteration, one workgroup, launched after col_merge:
+// the kernel boundary makes every block's partials and flags visible, where a
+// last-block ticket inside col_merge needed a device-scope fence in every
+// block (an L2 write-back and invalidate per block).  The error partials are
+// reduced in a fixed order (lane-strided sums, then a butterfly per wave, then
+// the waves in order), so the result is deterministic.
+__global__ __launch_bounds__(256) void col_decide_kernel(MergeArgs A) {
+  Ctrl* ctrl = A.ctrl;
+  if (ctrl->done || A.prologue) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int it = A.iter;
+  const bool col_first = A.order == kColFirst;
+  const bool check_col = col_first && (it % 10 == 0);
+  const bool check_row = !col_first && it >= 1 && ((it - 1) % 10 == 0);
+  const int nblk = (A.m + kMergeCols - 1) / kMergeCols;
+  __shared__ double wpart[4];
+  if (check_col || check_row) {
+    double tot = 0.0;
+    for (int q = w; q < A.B; q += 4) {
+      const double* src = check_col ? A.errpart + (int64_t)q * nblk : A.rowerr + (int64_t)q * A.G;
+      const int cnt = check_col ? nblk : A.G;
+      double acc = 0.0;
+      for (int b = lane; b < cnt; b += 64) acc += src[b];
+      acc = wave_sum(acc);
+      if (A.err_norm == 1) {
+        tot += acc;  // Frobenius norm over the batch
+      } else {
+        const double e = sqrt(acc);
+        tot = e > tot ? e : tot;  // maximum over the batch
+      }
+    }
+    if (lane == 0) wpart[w] = tot;
   }
   __syncthreads();
-  if (!last) return;
-  __threadfence();
   if (tid != 0) return;
-  ctrl->counter = 0;
-  if (A.prologue) return;
-  const int it = A.iter;
   const int cur_slot = it & 1;
   // Numerical guard of the scaling domain (ot/bregman/_sinkhorn.py:633-645).
   if (A.domain == kScaling && A.bad[cur_slot]) {
@@ -400,22 +476,13 @@ __global__ __launch_bounds__(256) void col_merge_kernel(MergeArgs A) {
     return;
   }
   if (check_col || check_row) {
-    double err = 0.0;
-    for (int q = 0; q < A.B; ++q) {
-      double acc = 0.0;
-      if (check_col) {
-        for (int b = 0; b < nblk; ++b) acc += A.errpart[(int64_t)q * nblk + b];
-      } else {
-        for (int b = 0; b < A.G; ++b) acc += A.rowerr[(int64_t)q * A.G + b];
-      }
-      if (A.err_norm == 1) {
-        err += acc;  // Frobenius norm over the batch
-      } else {
-        const double e = sqrt(acc);
-        err = e > err ? e : err;  // maximum over the batch
-      }
+    double err;
+    if (A.err_norm == 1) {
+      err = sqrt(((wpart[0] + wpart[1]) + wpart[2]) + wpart[3]);
+    } else {
+      err = wpart[0];
+      for (int k = 1; k < 4; ++k) err = wpart[k] > err ? wpart[k] : err;
     }
-    if (A.err_norm == 1) err = sqrt(err);
     ctrl->err = err;
     const int k = ctrl->n_err;
     if (k < A.err_cap) A.err_hist[k] = err;
@@ -625,6 +692,7 @@ hipError_t launch_col_merge(int dtype, const MergeArgs& a, int B, hipStream_t s)
     hipLaunchKernelGGL(col_merge_kernel<float>, grid, dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL(col_merge_kernel<double>, grid, dim3(256), 0, s, a);
+  if (!a.prologue) hipLaunchKernelGGL(col_decide_kernel, dim3(1), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
