@@ -171,14 +171,24 @@ def run_warmstart(args, torch, dev):
         # bf16 potential slot), 128 flops, and in the fused sweeps one exp2.
         secs = cold["kernel_ms"] * 1e-3
         tf = 128.0 * cold["pass_elements"] / secs / 1e12
+        # The binding unit is the VALU: per element of a fused sweep one exp2,
+        # one add (row sum) and one fma (column sum), measured on this GPU
+        # model at 8.2 + 2.6 + 2.7 cycles per wave64 instruction per SIMD
+        # (transcendentals do not co-issue with other VALU work), i.e. at most
+        # 4 SIMDs x 64 / 14.6 elements per clock per CU.
+        valu_peak = 256 * 4 * 64 / 14.6 * 2.4e9
         roof = {"bound": "mfma", "kernel": "batch_points_kernel (cold launch)",
                 "achieved": tf, "peak": MFMA_PEAK_TFLOPS["f16"], "unit": "TFLOP/s",
                 "frac": tf / MFMA_PEAK_TFLOPS["f16"], "traffic": None,
                 "flops_per_launch": 128.0 * cold["pass_elements"],
                 "avg_launch_ms": cold["kernel_ms"],
                 "exp2_per_s": cold["sweep_elements"] / secs,
+                "valu_bound": {"achieved": cold["sweep_elements"] / secs,
+                               "peak": valu_peak, "unit": "elements/s",
+                               "frac": cold["sweep_elements"] / secs / valu_peak},
                 "note": "executed MFMA flops of the cold launch; the points stay in LDS and "
-                        "registers, so HBM carries only points and potentials"}
+                        "registers, so HBM carries only points and potentials; the kernel "
+                        "is bound by the VALU (exp2 + add + fma per element), see valu_bound"}
     else:
         roof = {"bound": "hbm", "kernel": "batch_log_kernel (cold launch)",
                 "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

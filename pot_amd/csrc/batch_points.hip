@@ -324,6 +324,63 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     for (int j = 1; j < NS; ++j) sl += part[j];
     return halves_sum(sl);
   };
+  // tiles_ops + exp_rowsum of one row block, scheduled so the matrix core and
+  // the VALU work at the same time: the MFMA chains of the first two tiles,
+  // then the chains of the other tiles each followed by a share of the exp2
+  // and row-sum work of the first two, then the exp2 work of the others.
+  // (Both waves of a SIMD meet the block barrier in step, so the other wave
+  // cannot fill one unit while this one works on the other.)  Same
+  // arithmetic, in the same order, as tiles_ops followed by exp_rowsum.
+  auto sweep_block = [&](u32x4 bh, u32x4 bl, u32x4 bk) {
+    const f32x16 z = {};
+    constexpr int NA = NS < 2 ? NS : 2;  // tiles whose chains open the block
+    constexpr int NB = NS - NA;          // tiles whose chains overlap exp2 work
+    auto mf = [&](int j, int k) {
+      if (k == 0) e[j] = mfma_h(alo[j], bh, z);
+      if (k == 1) e[j] = mfma_h(ahi[j], bl, e[j]);
+      if (k == 2) e[j] = mfma_h(ahi[j], bh, e[j]);
+      if (k == 3) e[j] = mfma_b(ak[j], bk, e[j]);
+    };
+    float part[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) part[j] = 0.f;
+    auto ex = [&](int j, int i) {
+      e[j][i] = exp2_fast(e[j][i]);
+      part[j] += e[j][i];
+    };
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int j = 0; j < NA; ++j) mf(j, k);
+    __builtin_amdgcn_sched_group_barrier(0x008, 4 * NA, 0);
+    if constexpr (NB > 0) {
+      constexpr int PER = 16 * NA / (4 * NB);  // exp2 elements per MFMA
+#pragma unroll
+      for (int q = 0; q < 4 * NB; ++q) {
+        mf(NA + q % NB, q / NB);
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+          const int f = q * PER + t;
+          ex(f / 16, f % 16);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 2 * PER, 0);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NA; ++j)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ex(j, i);
+    }
+#pragma unroll
+    for (int j = NA; j < NS; ++j)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) ex(j, i);
+    float sl = part[0];
+#pragma unroll
+    for (int j = 1; j < NS; ++j) sl += part[j];
+    return halves_sum(sl);
+  };
   auto row_kslot = [&](int blk) {
     const u32x4 k = L.rks[blk * 32 + l32];
     return h == 0 ? k : zero4;
@@ -492,13 +549,14 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     u32x4 nbh = L.rop[0][0][lane], nbl = L.rop[0][1][lane], nbk = row_kslot(0);
     for (int blk = 0; blk < nblk; ++blk) {
       u32x4 bk = nbk;
-      tiles_ops(nbh, nbl, bk);
-      if (blk + 1 < nblk) {
-        nbh = L.rop[blk + 1][0][lane];
-        nbl = L.rop[blk + 1][1][lane];
-        nbk = row_kslot(blk + 1);
-      }
-      float sl = exp_rowsum();
+      const u32x4 bh = nbh, bl = nbl;
+      // Unconditional (the last block re-reads itself), so the block's MFMA
+      // and exp2 work stay in one scheduling region.
+      const int nb = blk + 1 < nblk ? blk + 1 : blk;
+      nbh = L.rop[nb][0][lane];
+      nbl = L.rop[nb][1][lane];
+      nbk = row_kslot(nb);
+      float sl = sweep_block(bh, bl, bk);
       const int buf = blk & 1;
       if (h == 0) L.rowpart[buf][wave][l32] = sl;
       __syncthreads();
