@@ -1,18 +1,26 @@
-"""Tile-shape autotuner of the dense solver.
+"""Tile-shape autotuner of the dense solver and of the ragged batch solvers.
 
-For a (dtype, n, m) shape class it times every compiled row-sweep
+Dense: for a (dtype, n, m) shape class it times every compiled row-sweep
 configuration that covers m -- threads per workgroup, columns per thread,
 rows prefetched, rows per workgroup -- on a few fixed iterations of a
-synthetic problem of that class, and keeps the fastest.  Results are cached
-in a JSON file keyed by GPU architecture and shape class; the solver looks
-the cache up and falls back to the library's default shape when it has no
-entry.
+synthetic problem of that class, and keeps the fastest.
+
+Ragged batches: for a (cost kind, largest n, largest m, d) class it times
+every compiled batch configuration -- wavefronts per problem, and for
+materialised costs rows prefetched and the register cap -- on a few fixed
+iterations of a synthetic batch with the caller's sizes.
+
+Results are cached in a JSON file keyed by GPU architecture and shape class;
+the solvers look the cache up and fall back to the library's default shape
+when it has no entry.
 """
 
 import json
 import os
 import threading
 import time
+
+import numpy as np
 
 import torch
 
@@ -117,8 +125,108 @@ def tune(n, m, dtype="f32", iters=20, reps=2, save=True, log=None):
     return best_tile, best_t
 
 
+# Ragged batch configurations compiled in csrc/batch.hip (waves, rows
+# prefetched, register cap in waves per SIMD; 255 = no cap) and
+# csrc/batch_points.hip (waves).
+BATCH_CONFIGS = {
+    "dense": [(8, 1, 4), (8, 1, 255), (4, 1, 0), (8, 2, 0), (4, 2, 0)],
+    "points": [(8, 0, 0), (16, 0, 0)],
+}
+
+
+def _pow2(x):
+    """Nearest power of two (in the log) of a positive number."""
+    return 1 << max(0, int(round(np.log2(max(float(x), 1.0)))))
+
+
+def _batch_key(cost, ns, ms, d):
+    """Largest sizes (rounded up) and mean sizes (rounded) of the batch."""
+    mean = f"{_pow2(np.mean(ns))}x{_pow2(np.mean(ms))}"
+    return f"{_arch()}|batch-{cost}|{_class(max(ns), max(ms))}|mean{mean}|d{int(d)}"
+
+
+def lookup_batch(cost, ns, ms, d=0):
+    """Cached (waves, prefetch, occupancy) of a ragged batch class, or None."""
+    if len(ns) == 0:
+        return None
+    ent = _load().get(_batch_key(cost, ns, ms, d if cost == "points" else 0))
+    return tuple(ent["config"]) if ent else None
+
+
+def tune_batch(ns, ms, d=16, cost="points", reg=1e-2, iters=20, reps=2, save=True, log=None,
+               seed=0):
+    """Time every batch configuration on ``iters`` fixed iterations of
+    synthetic Gaussian clouds with these sizes; store and return the best."""
+    from . import ragged
+    if cost not in BATCH_CONFIGS:
+        raise ValueError("cost must be 'dense' or 'points'")
+    rng = np.random.default_rng(seed)
+    ns = [int(n) for n in ns]
+    ms = [int(m) for m in ms]
+    xs = [rng.standard_normal((n, d)).astype(np.float32) for n in ns]
+    ys = [(rng.standard_normal((m, d)) + 0.5).astype(np.float32) for m in ms]
+    regs = np.full(len(ns), float(reg))
+    if cost == "points":
+        P = ragged.points_from_clouds(xs, ys, regs)
+
+        def run(cfg):
+            ragged.sinkhorn_log_ragged_points(P, numItermax=iters, stopThr=0.0, waves=cfg[0])
+    else:
+        R = ragged.from_clouds(xs, ys, regs)
+
+        def run(cfg):
+            ragged.sinkhorn_log_ragged(R, numItermax=iters, stopThr=0.0, waves=cfg[0],
+                                       prefetch=cfg[1], occupancy=cfg[2])
+    results = []
+    for cfg in BATCH_CONFIGS[cost]:
+        best = float("inf")
+        try:
+            run(cfg)  # warm-up
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                run(cfg)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+        except _lib.LibraryError:
+            continue
+        results.append((best, cfg))
+        if log:
+            log(f"  batch config {cfg}: {best * 1e3:.2f} ms for {iters} iterations")
+    results.sort()
+    best_t, best_cfg = results[0]
+    if save:
+        with _lock:
+            mem = _load()
+            mem[_batch_key(cost, ns, ms, d if cost == "points" else 0)] = {
+                "config": list(best_cfg), "ms": best_t * 1e3, "iters": iters,
+                "batch": len(ns), "measured": [[t * 1e3, list(c)] for t, c in results]}
+            with open(CACHE, "w") as fh:
+                json.dump(mem, fh, indent=1, sort_keys=True)
+    return best_cfg, best_t
+
+
+def _main_batch(argv):
+    """python -m pot_amd.autotune batch COST B NMIN NMAX D: sizes uniform in
+    [NMIN, NMAX] (seed 0), as in the ragged and warm-start workloads."""
+    cost = argv[0] if argv else "points"
+    B = int(argv[1]) if len(argv) > 1 else 4096
+    lo = int(argv[2]) if len(argv) > 2 else 1024
+    hi = int(argv[3]) if len(argv) > 3 else lo
+    d = int(argv[4]) if len(argv) > 4 else 16
+    rng = np.random.default_rng(0)
+    ns = rng.integers(lo, hi + 1, size=B)
+    ms = rng.integers(lo, hi + 1, size=B)
+    cfg, t = tune_batch(ns, ms, d=d, cost=cost, log=lambda s: print(s, flush=True))
+    print(json.dumps({"cost": cost, "batch": B, "sizes": [lo, hi], "d": d, "config": list(cfg),
+                      "ms": t * 1e3}))
+
+
 if __name__ == "__main__":
     import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "batch":
+        _main_batch(sys.argv[2:])
+        sys.exit(0)
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 16384
     m = int(sys.argv[2]) if len(sys.argv) > 2 else n
     dt = sys.argv[3] if len(sys.argv) > 3 else "f32"

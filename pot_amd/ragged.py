@@ -29,6 +29,17 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def _tuned(cost, ns, ms, d, waves, prefetch=0, occupancy=0):
+    """The caller's batch configuration, or the autotuner's for this batch
+    class when the caller leaves all of it at 0 (then the library default if
+    the class was never tuned)."""
+    if waves or prefetch or occupancy:
+        return int(waves), int(prefetch), int(occupancy)
+    from . import autotune
+    cfg = autotune.lookup_batch(cost, ns, ms, d)
+    return tuple(int(c) for c in cfg) if cfg else (0, 0, 0)
+
+
 def _round4(x):
     return (x + 3) // 4 * 4
 
@@ -206,6 +217,7 @@ def sinkhorn_log_ragged(R, u0=None, numItermax=1000, stopThr=1e-9, order=None,
     e_last = torch.empty(B, dtype=torch.float64, device=dev)
     if order is None:
         order = launch_order(R)
+    waves, prefetch, occupancy = _tuned("dense", R.ns, R.ms, 0, waves, prefetch, occupancy)
     rc = lib.potamd_sinkhorn_log_batch(
         _lib.F32, B, int(R.ns.max()), int(R.ms.max()), _ptr(R.M), _ptr(R.moff), _ptr(R.ns_d),
         _ptr(R.ms_d), _ptr(R.ldms_d), _ptr(R.aoff), _ptr(R.boff), _ptr(R.regs), _ptr(R.a),
@@ -333,7 +345,8 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
     its maximum when P.normalize), evaluated on MFMA from the points in every
     sweep and never stored.  ``cmax``: the maxima from ``points_cmax`` (the
     solve then skips its own search).  ``waves``: wavefronts per problem (8
-    or 16; 0 = the library default).  Returns a RaggedResult whose ``cmax``
+    or 16; 0 = the autotuned value of this batch class, else the library
+    default).  Returns a RaggedResult whose ``cmax``
     holds the cost maxima; asynchronous on the current stream.
     """
     lib = _lib.load()
@@ -354,6 +367,7 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
         order = torch.from_numpy(_points_order(P.ns, P.ms, regs_h)).to(dev)
     else:
         order = torch.as_tensor(_check_order(order, B), device=dev)
+    waves = _tuned("points", P.ns, P.ms, P.d, waves)[0]
     # work-queue counters of the persistent launch (the kernel leaves them zero)
     queue = torch.zeros(2, dtype=torch.int32, device=dev)
     _launch_points(lib, P, P.X, P.Y, P.regs, u, v, niter, status, e_last, e_first, cmax, order,
@@ -414,6 +428,9 @@ class RaggedGraph:
         self.e_last = torch.zeros(B, dtype=torch.float64, device=dev)
         self.order = torch.arange(B, dtype=torch.int32, device=dev)
         self.queue = torch.zeros(2, dtype=torch.int32, device=dev)  # left zero by each launch
+        waves, prefetch, occupancy = _tuned(cost, self.ns, self.ms,
+                                            self.d if cost == "points" else 0,
+                                            waves, prefetch, occupancy)
         self.settings = (int(numItermax), int(check_every), float(stopThr), float(fallback_thr),
                          int(waves) | (int(prefetch) << 8) | (int(occupancy) << 16))
         # eager warm-up on a side stream, then capture
@@ -437,11 +454,11 @@ class RaggedGraph:
         stream = ctypes.c_void_p(_device.stream_ptr(dev))
         if self.cost == "points":
             self.u.copy_(self.u0)
-            max_iter, check_every, stop, fb, _ = self.settings
+            max_iter, check_every, stop, fb, wv = self.settings
             _launch_points(self.lib, self, self.X, self.Y, self.reg_in, self.u, self.v,
                            self.niter, self.status, self.e_last, self.e_first, self.cmax,
                            self.order, max_iter, check_every, stop, fb, stream,
-                           queue=self.queue)
+                           waves=wv & 0xff, queue=self.queue)
             return
         rc = self.lib.potamd_sqeuclidean_ragged(
             len(self.ns), int(self.ns.max()), int(self.ms.max()), self.d, _ptr(self.X),

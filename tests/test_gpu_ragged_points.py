@@ -231,3 +231,32 @@ def test_points_graph_replays_match_eager():
         assert np.array_equal(out.u.cpu().numpy(), ref_u)
         assert np.array_equal(out.v.cpu().numpy(), ref_v)
         _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+
+
+def test_batch_autotuner_picks_a_compiled_config(tmp_path, monkeypatch):
+    from pot_amd import autotune
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_mem", None)
+    fam = problems.ragged_family(12, seed=21, n_range=(64, 300), d=8)
+    ns = [len(x) for x in fam.xs]
+    ms = [len(y) for y in fam.ys]
+    for cost in ("points", "dense"):
+        print("tuning", cost, flush=True)
+        cfg, _ = autotune.tune_batch(ns, ms, d=8, cost=cost, iters=5, reps=1)
+        assert tuple(cfg) in autotune.BATCH_CONFIGS[cost]
+        assert autotune.lookup_batch(cost, ns, ms, 8 if cost == "points" else 0) == tuple(cfg)
+    # a solve that leaves the configuration at 0 runs the tuned one; its plans
+    # agree with the library default's
+    P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
+    rt = ragged.sinkhorn_log_ragged_points(P, numItermax=300, stopThr=1e-7, check_every=10)
+    r8 = ragged.sinkhorn_log_ragged_points(P, numItermax=300, stopThr=1e-7, check_every=10,
+                                           waves=8)
+    torch.cuda.synchronize()
+    print("tuned and default solves done", flush=True)
+    for p in range(P.batch):
+        M = _cost(fam.xs[p], fam.ys[p])
+        ut, vt = rt.split(P, p)
+        u8, v8 = r8.split(P, p)
+        Pt = _plan(M, ut.cpu().numpy(), vt.cpu().numpy(), fam.regs[p])
+        P8 = _plan(M, u8.cpu().numpy(), v8.cpu().numpy(), fam.regs[p])
+        assert np.max(np.abs(Pt - P8)) / P8.max() < 1e-5, p

@@ -95,3 +95,20 @@ def test_no_gpu_means_loud_failure():
     import pot_amd
     with pytest.raises(RuntimeError, match="ROCm GPU"):
         pot_amd.sinkhorn([0.5, 0.5], [0.5, 0.5], np.eye(2), 1.0)
+
+
+def test_batch_tuner_key_and_lookup(monkeypatch):
+    import numpy as np
+
+    from pot_amd import autotune, ragged
+    ns = np.full(8, 1000)
+    ms = np.full(8, 700)
+    key = autotune._batch_key("points", ns, ms, 16)
+    assert key.endswith("|batch-points|1024x1024|mean1024x512|d16")
+    monkeypatch.setattr(autotune, "_mem", {key: {"config": [16, 0, 0]}})
+    assert autotune.lookup_batch("points", ns, ms, 16) == (16, 0, 0)
+    assert ragged._tuned("points", ns, ms, 16, 0) == (16, 0, 0)
+    # an explicit configuration wins; an untuned class gets the library default
+    assert ragged._tuned("points", ns, ms, 16, 8) == (8, 0, 0)
+    assert ragged._tuned("dense", ns, ms, 0, 0) == (0, 0, 0)
+    assert autotune.lookup_batch("points", [], [], 16) is None
