@@ -647,19 +647,28 @@ struct OnlineMergeArgs {
   Ctrl* ctrl;
 };
 
-__global__ __launch_bounds__(256) void online_merge_kernel(OnlineMergeArgs A) {
+// One wave per workgroup (the R side has 16384 points at the headline size:
+// 256 workgroups fill the chip), the split partials read 8 at a time so their
+// loads overlap.  The stop decision is online_decide_kernel's, launched after
+// this kernel, so no block needs a device-scope fence.
+constexpr int kMergeRows = 64;
+
+__global__ __launch_bounds__(kMergeRows) void online_merge_kernel(OnlineMergeArgs A) {
   Ctrl* ctrl = A.ctrl;
   if (ctrl->done) return;
   const int tid = threadIdx.x;
-  const int r = blockIdx.x * 256 + tid;
+  const int r = blockIdx.x * kMergeRows + tid;
   double e2 = 0.0;
   if (r < A.nr) {
+    const double* pr = A.part + r;
     double mx = -__builtin_inf();
-    for (int s = 0; s < A.S; ++s) mx = fmax(mx, A.part[(int64_t)s * A.nr_pad + r]);
+#pragma unroll 8
+    for (int s = 0; s < A.S; ++s) mx = fmax(mx, pr[(int64_t)s * A.nr_pad]);
     double I = -__builtin_inf();
     if (mx > -__builtin_inf()) {
       double acc = 0.0;
-      for (int s = 0; s < A.S; ++s) acc += exp2(A.part[(int64_t)s * A.nr_pad + r] - mx);
+#pragma unroll 8
+      for (int s = 0; s < A.S; ++s) acc += exp2(pr[(int64_t)s * A.nr_pad] - mx);
       I = mx + log2(acc);
     }
     const double s2 = s2_of(A.cscale, A.reg);
@@ -672,24 +681,31 @@ __global__ __launch_bounds__(256) void online_merge_kernel(OnlineMergeArgs A) {
     A.pot_new[r] = wr > 0.0 ? A.logw2[r] - L2 : -__builtin_inf();
     A.shift[r] = I;
   }
-  if (!A.decide) return;
-  __shared__ double red[4];
-  const double s = wave_sum(e2);
-  if ((tid & 63) == 0) red[tid >> 6] = s;
-  __syncthreads();
-  if (tid == 0) A.errpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-  __threadfence();
-  __shared__ int last;
-  if (tid == 0) last = atomicAdd(&ctrl->counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (tid != 0) return;
-  ctrl->counter = 0;
-  const int it = A.iter;
+  if (!A.check) return;
+  e2 = wave_sum(e2);
+  if (tid == 0) A.errpart[blockIdx.x] = e2;
+}
+
+// Stop decision after a checking (or the last) merge: one workgroup; the
+// kernel boundary makes the merge's partials visible.  The partials are
+// summed in a fixed order (lane-strided, a butterfly per wave, the waves in
+// order), so the result is deterministic.
+__global__ __launch_bounds__(256) void online_decide_kernel(OnlineMergeArgs A, int nblk) {
+  Ctrl* ctrl = A.ctrl;
+  if (ctrl->done) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  __shared__ double wpart[4];
   if (A.check) {
     double acc = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b) acc += A.errpart[b];
+    for (int b = tid; b < nblk; b += 256) acc += A.errpart[b];
+    acc = wave_sum(acc);
+    if (lane == 0) wpart[w] = acc;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  const int it = A.iter;
+  if (A.check) {
+    const double acc = ((wpart[0] + wpart[1]) + wpart[2]) + wpart[3];
     const double err = A.l1 ? acc : sqrt(acc);
     ctrl->err = err;
     const int k = ctrl->n_err;
@@ -946,7 +962,7 @@ OnlineLayout carve_online(void* ws, int prec, int n, int m, int d) {
   const size_t parts = (size_t)64 * std::max(np, mp);
   L.part = c.take<double>(parts);
   L.pidx = c.take<int>(parts);
-  L.errpart = c.take<double>((std::max(n, m) + 255) / 256);
+  L.errpart = c.take<double>((std::max(n, m) + kMergeRows - 1) / kMergeRows);
   L.blockmax = c.take<double>((std::max(n, m) + 255) / 256);
   L.cscale = c.take<double>(1);
   L.ctrl = c.take<Ctrl>(1);
@@ -1068,7 +1084,9 @@ hipError_t half_step(int prec, int D, int d, const Side& R, const Side& C, const
   ma.reg = reg;
   ma.errpart = L.errpart;
   ma.ctrl = L.ctrl;
-  hipLaunchKernelGGL(online_merge_kernel, dim3((R.count + 255) / 256), dim3(256), 0, s, ma);
+  const int nblk = (R.count + kMergeRows - 1) / kMergeRows;
+  hipLaunchKernelGGL(online_merge_kernel, dim3(nblk), dim3(kMergeRows), 0, s, ma);
+  if (ma.decide) hipLaunchKernelGGL(online_decide_kernel, dim3(1), dim3(256), 0, s, ma, nblk);
   return hipGetLastError();
 }
 
