@@ -160,10 +160,10 @@ def run_warmstart(args, torch, dev):
     # The roofline describes the cold launch of batch_log_kernel: the kernel
     # is the same for both starts, and its HBM traffic was measured with
     # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this exact (seeded,
-    # deterministic) cold workload: 3.3667e8 KiB fetched x 2.000 (the counter
-    # scale of a 1 GiB device copy) + 2.295e5 KiB written x 1.000.
+    # deterministic) cold workload: 3.3735e8 KiB fetched x 2.000 (the counter
+    # scale of a 1 GiB device copy) + 6.610e4 KiB written x 1.000.
     gbs = cold["alg_bytes"] / (cold["kernel_ms"] * 1e-3) / 1e9
-    traffic = 3.3667e8 * 1024 * 2.000 + 2.295e5 * 1024 * 1.000 \
+    traffic = 3.3735e8 * 1024 * 2.000 + 6.610e4 * 1024 * 1.000 \
         if (B == 4096 and args.stop == 1e-6 and args.max_iter == 2000) else None
     if args.cost == "points":
         # The point-cloud kernel streams no cost: per element of every pass it

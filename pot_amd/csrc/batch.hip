@@ -70,26 +70,65 @@ void batch_log_kernel(BatchArgs A) {
   }
   __syncthreads();
 
+  // Every load is unconditional: the compiler's wait counts assume the fewest
+  // loads any path may have issued, so a load under a condition (or whose
+  // value is consumed inside one) is waited for at once.  A lane past the last
+  // column loads the row's first columns; mask_row zeroes those values (and
+  // the row padding past m, which may hold anything) where they are used.
   auto load_row = [&](int i, float* dst) {
     const float* row = Mp + (int64_t)i * ldm;
 #pragma unroll
     for (int k = 0; k < NCH; ++k) {
       const int j0 = (k * kWave + lane) * VEC;
-      if (j0 < m) {
-        if constexpr (VEC == 4) {
-          typedef float f4 __attribute__((ext_vector_type(4)));
-          f4 x = __builtin_nontemporal_load(reinterpret_cast<const f4*>(row + j0));
-          // Columns past m (row padding) may hold anything: zero them.
-          dst[k * 4 + 0] = x.x;
-          dst[k * 4 + 1] = j0 + 1 < m ? x.y : 0.f;
-          dst[k * 4 + 2] = j0 + 2 < m ? x.z : 0.f;
-          dst[k * 4 + 3] = j0 + 3 < m ? x.w : 0.f;
-        } else {
-          dst[k] = __builtin_nontemporal_load(row + j0);
-        }
+      const int jl = j0 < m ? j0 : 0;
+      if constexpr (VEC == 4) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const f4 x = __builtin_nontemporal_load(reinterpret_cast<const f4*>(row + jl));
+        dst[k * 4 + 0] = x.x;
+        dst[k * 4 + 1] = x.y;
+        dst[k * 4 + 2] = x.z;
+        dst[k * 4 + 3] = x.w;
       } else {
+        dst[k] = __builtin_nontemporal_load(row + jl);
+      }
+    }
+  };
+  const bool padded = m < MMAX;
+  auto mask_row = [&](float* cur) {
+    if (padded) {
 #pragma unroll
-        for (int c = 0; c < VEC; ++c) dst[k * VEC + c] = 0.f;
+      for (int k = 0; k < NCH; ++k)
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) {
+          const int j = (k * kWave + lane) * VEC + c;
+          cur[k * VEC + c] = j < m ? cur[k * VEC + c] : 0.f;
+        }
+    }
+  };
+  // This wave's rows wid, wid + W, ... streamed through PF + 1 buffers: while
+  // row k is processed, the next PF rows are in flight.  The loop is unrolled
+  // over the buffers so they rotate without register copies (a copy would
+  // wait for the prefetch); a prefetch past the wave's last row reads that
+  // row again.  body(i, cur) gets the row's logical index and its values.
+  constexpr int NB = PF + 1;
+  auto for_rows = [&](auto phys, auto body) {
+    const int nl = wid < n ? (n - wid + W - 1) / W : 0;
+    if (nl == 0) return;
+    float buf[NB][CPT];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      load_row(phys(wid + (b < nl ? b : nl - 1) * W), buf[b]);
+      __builtin_amdgcn_sched_barrier(0);  // issue the buffers in order
+    }
+    for (int k0 = 0; k0 < nl; k0 += NB) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int k = k0 + b;
+        if (k < nl) {
+          mask_row(buf[b]);
+          body(wid + k * W, buf[b]);
+        }
+        load_row(phys(wid + (k + NB < nl ? k + NB : nl - 1) * W), buf[b]);
       }
     }
   };
@@ -107,9 +146,7 @@ restart:
       mx[q] = -__builtin_inff();
       sm[q] = 0.f;
     }
-    float cur[CPT];
-    for (int i = wid; i < n; i += W) {
-      load_row(i, cur);
+    for_rows([](int r) { return r; }, [&](int i, float* cur) {
       const float ui = (float)u2[i];
 #pragma unroll
       for (int q = 0; q < CPT; ++q) {
@@ -120,7 +157,7 @@ restart:
         sm[q] = up_ ? __builtin_fmaf(sm[q], e, 1.f) : sm[q] + e;
         mx[q] = up_ ? x : mx[q];
       }
-    }
+    });
 #pragma unroll
     for (int q = 0; q < CPT; ++q)
       part[wid][q * kWave + lane] =
@@ -142,8 +179,6 @@ restart:
     __syncthreads();
   }
 
-  // Rows in flight per wave: the current one plus PF prefetched.
-  float cur[CPT], nxt[CPT], nx2[PF > 1 ? CPT : 1];
   for (int ii = 0; ii < A.max_iter; ++ii) {
     // ---- row sweep: u from v, column sums of P(u_new, v) ----
     float acc[CPT];
@@ -154,17 +189,7 @@ restart:
     // side do not hit the same address offsets at the same time.
     const int rot = (A.rotate && n > 0) ? (int)(((int64_t)p * 337) % n) : 0;
     auto pr = [&](int r) { const int q = r + rot; return q < n ? q : q - n; };
-    int i = wid;
-    if (i < n) load_row(pr(i), cur);
-    if constexpr (PF > 1) {
-      if (i + W < n) load_row(pr(i + W), nxt);
-    }
-    for (; i < n; i += W) {
-      if constexpr (PF > 1) {
-        if (i + 2 * W < n) load_row(pr(i + 2 * W), nx2);
-      } else {
-        if (i + W < n) load_row(pr(i + W), nxt);
-      }
+    for_rows(pr, [&](int i, float* cur) {
       const int ip = pr(i);
       const float a_i = af[ip];
       if (a_i > 0.f) {
@@ -179,6 +204,7 @@ restart:
         float S = wave_sum_dpp(s);
         if (!in_safe_range((double)S)) {
           load_row(ip, cur);
+          mask_row(cur);
           float mxr = -__builtin_inff();
 #pragma unroll
           for (int q = 0; q < CPT; ++q) {
@@ -203,13 +229,7 @@ restart:
 #pragma unroll
         for (int q = 0; q < CPT; ++q) acc[q] = __builtin_fmaf(cur[q], kf, acc[q]);
       }
-#pragma unroll
-      for (int q = 0; q < CPT; ++q) cur[q] = nxt[q];
-      if constexpr (PF > 1) {
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) nxt[q] = nx2[q];
-      }
-    }
+    });
 #pragma unroll
     for (int q = 0; q < CPT; ++q) part[wid][q * kWave + lane] = acc[q];
     __syncthreads();
