@@ -32,10 +32,12 @@ CFLAGS = [
 ]
 
 
-# Per-file flags.  The online-cost pass keeps its MFMA accumulators in VGPRs,
-# so its exp2 epilogue reads them without accvgpr copies.
+# Per-file flags.  The online-cost pass and the fused network kernels keep
+# their MFMA accumulators in VGPRs, so their epilogues read them without
+# accvgpr copies.
 EXTRA = {
     "online.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+    "mlp_fused.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
 }
 
 

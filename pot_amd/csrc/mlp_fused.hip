@@ -79,7 +79,8 @@ __device__ __forceinline__ unsigned pack_bf2(float lo, float hi) {
   return __builtin_bit_cast(unsigned, v);
 }
 __device__ __forceinline__ float round_bf(float x) { return (float)(__bf16)x; }
-__device__ __forceinline__ float leaky(float x) { return x > 0.f ? x : kFLeak * x; }
+// Leaky ReLU with a slope below 1: max(x, slope x) (two VALU operations).
+__device__ __forceinline__ float leaky(float x) { return fmaxf(x, kFLeak * x); }
 
 __device__ __forceinline__ f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
@@ -112,7 +113,7 @@ struct BlockIn<0> {  // given bf16 rows
         raw[t][s] = *reinterpret_cast<const u32x4*>(A.inp + (r0 + 32 * t + l32) * kFK + 16 * s +
                                                     8 * h);
   }
-  __device__ void make(const FusedMlpArgs&, int, int, u32x4 (&x)[2][2]) const {
+  __device__ void make(const FusedMlpArgs&, int, float, float, u32x4 (&x)[2][2]) const {
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -134,10 +135,9 @@ struct BlockIn<1> {  // built from the points
       w[t] = A.wts[r];
     }
   }
-  // seg: the block's segment (scale and log reg are per segment).
-  __device__ void make(const FusedMlpArgs& A, int seg, int h, u32x4 (&x)[2][2]) const {
-    const float scale = rsqrtf(A.cmax[seg]);
-    const float lr = (float)((log(A.regs[seg]) - kLogRegCenter) / kLogRegScale);
+  // scale, lr: the segment's point scale and normalised log reg.
+  __device__ void make(const FusedMlpArgs& A, int h, float scale, float lr,
+                       u32x4 (&x)[2][2]) const {
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
 #pragma unroll
@@ -174,6 +174,22 @@ void mlp_fused_kernel(FusedMlpArgs A) {
   }
   // In mode 0 a lane owns column jbase + 32 c + l32 of layer 2.
   const float b2c[2] = {A.b2[jbase + l32], A.b2[jbase + 32 + l32]};
+  // Mode 1: the layer-2 bias and the head weights of the lane's values
+  // (column jbase + 32 c + 8 q + 4 h + i), kept in registers: a global load
+  // inside the block loop would be waited for there, with one wave per SIMD.
+  float b2v[2][4][4], w3v[2][4][4];
+  if constexpr (MODE == 1) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int j = jbase + 32 * c + 8 * q + 4 * h + i;
+          b2v[c][q][i] = A.b2[j];
+          w3v[c][q][i] = A.w3[j];
+        }
+  }
 
   const int nseg = A.rows / A.seg_len;
   const int bps = A.seg_len / kFRows;
@@ -182,19 +198,35 @@ void mlp_fused_kernel(FusedMlpArgs A) {
   int buf = 0;
   for (int p = blockIdx.x; p < nseg; p += gridDim.x) {
     float zacc[2] = {0.f, 0.f};
+    // Per-segment values, once per segment: the layer-1 bias of the lane's
+    // values and (inputs built from the points) the scale and log reg.
     const float* bias1 = MODE == 0 ? A.bias1 : A.bias1 + (int64_t)p * kFH;
+    float b1v[2][4][4];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b1v[c][q][i] = bias1[jbase + 32 * c + 8 * q + 4 * h + i];
+    float seg_scale = 1.f, seg_lr = 0.f;
+    if constexpr (IN == 1) {
+      seg_scale = rsqrtf(A.cmax[p]);
+      seg_lr = (float)((log(A.regs[p]) - kLogRegCenter) / kLogRegScale);
+    }
     for (int blk = 0; blk < bps; ++blk) {
       const int64_t r0 = (int64_t)p * A.seg_len + (int64_t)blk * kFRows;
       u32x4 xin[2][2];
-      nxt.make(A, p, h, xin);
-      // Fetch the next block (this segment's next, or the next segment's first).
+      nxt.make(A, h, seg_scale, seg_lr, xin);
+      // Fetch the next block (this segment's next, or the next segment's
+      // first; after the last block this one again, so the fetch is
+      // unconditional and its wait is not pulled forward).
       {
-        int64_t rn = -1;
+        int64_t rn = r0;
         if (blk + 1 < bps)
           rn = r0 + kFRows;
         else if (p + (int)gridDim.x < nseg)
           rn = (int64_t)(p + gridDim.x) * A.seg_len;
-        if (rn >= 0) nxt.fetch(A, rn, h, l32);
+        nxt.fetch(A, rn, h, l32);
       }
       // ---- layer 1, transposed: D[j][r], r = 32 t + l32 ----
 #pragma unroll
@@ -210,7 +242,7 @@ void mlp_fused_kernel(FusedMlpArgs A) {
             const int j0 = jbase + 32 * c + 8 * q + 4 * h;
             float v[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = leaky(acc[4 * q + i] + bias1[j0 + i]);
+            for (int i = 0; i < 4; ++i) v[i] = leaky(acc[4 * q + i] + b1v[c][q][i]);
             u32x2 pk;
             pk[0] = pack_bf2(v[0], v[1]);
             pk[1] = pack_bf2(v[2], v[3]);
@@ -255,11 +287,10 @@ void mlp_fused_kernel(FusedMlpArgs A) {
           for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              const int j = jbase + 32 * c + 8 * q + 4 * h + i;
-              const float bj = A.b2[j], wj = A.w3[j];
 #pragma unroll
               for (int t = 0; t < 2; ++t)
-                fp[t] = __builtin_fmaf(round_bf(leaky(acc[c][t][4 * q + i] + bj)), wj, fp[t]);
+                fp[t] = __builtin_fmaf(round_bf(leaky(acc[c][t][4 * q + i] + b2v[c][q][i])),
+                                       w3v[c][q][i], fp[t]);
             }
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
