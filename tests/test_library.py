@@ -112,3 +112,41 @@ def test_batch_tuner_key_and_lookup(monkeypatch):
     assert ragged._tuned("points", ns, ms, 16, 8) == (8, 0, 0)
     assert ragged._tuned("dense", ns, ms, 0, 0) == (0, 0, 0)
     assert autotune.lookup_batch("points", [], [], 16) is None
+
+
+def test_points_and_network_argument_validation():
+    # invalid arguments are rejected before any device call
+    lib = _lib.load()
+    P = ctypes.c_void_p(0x1000)
+    Q = ctypes.c_void_p(0x1008)  # not 16-byte aligned
+
+    def pts(**kw):
+        a = dict(batch=1, max_n=8, max_m=8, d=2, normalize=1, max_iter=10, check=1, waves=0,
+                 cmax=P, queue=None)
+        a.update(kw)
+        return lib.potamd_sinkhorn_log_batch_points(
+            a["batch"], a["max_n"], a["max_m"], a["d"], P, P, P, P, P, P, P, a["normalize"],
+            P, P, P, P, a["max_iter"], a["check"], 1e-9, 0.0, None, P, P, P, P, a["cmax"],
+            a["waves"], a["queue"], None)
+
+    assert pts(batch=0) == -1
+    assert pts(max_iter=0) == -1
+    assert pts(normalize=3) == -1
+    assert pts(normalize=2, cmax=None) == -1
+    assert pts(max_n=4096) == -3
+    assert pts(d=17) == -3
+    assert pts(waves=12) == -1
+    assert lib.potamd_ragged_points_cmax(1, 8, 8, 2, P, P, P, P, P, P, None, None) == -1
+    assert lib.potamd_ragged_points_cmax(1, 8, 8, 64, P, P, P, P, P, P, P, None) == -3
+
+    def mlp(**kw):
+        a = dict(mode=0, rows=1024, seg=1024, inp=P, pts=None)
+        a.update(kw)
+        return lib.potamd_mlp_fused(a["mode"], a["rows"], a["seg"], a["inp"], a["pts"], None,
+                                    None, None, P, P, P, P, P, P, None, P, None)
+
+    assert mlp(mode=2) == -1
+    assert mlp(rows=0) == -1
+    assert mlp(seg=100, rows=1000) == -3     # segments must be whole 64-row blocks
+    assert mlp(inp=None) == -1               # no input rows and no points
+    assert mlp(inp=Q) == -1                  # 16-byte loads need alignment
