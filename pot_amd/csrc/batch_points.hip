@@ -214,6 +214,8 @@ __device__ __forceinline__ float half_reduce(float x) {
   return op(x, y);
 }
 
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
 template <int PW>
 struct PointsLds {
   u32x4 rop[kPBlocks][2][kWave];  // row operands: [block][hi, lo][lane]
@@ -226,7 +228,9 @@ struct PointsLds {
   float af[kPMax];                // row weights
   float sh[kPMax];                // shifted row sums (prologue: column maxima)
   float colx[kPMax];              // column sums
-  float rowpart[2][PW][32];
+  // Row-sum partials [buffer][row][wave]: a row's PW partials are adjacent,
+  // so a lane reads them with 16-byte loads in one LDS round trip.
+  __attribute__((aligned(16))) float rowpart[2][32][PW];
   float rowmax[PW][32];
   double cen[kPD];
   double red[PW];
@@ -380,6 +384,20 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
 #pragma unroll
     for (int j = 1; j < NS; ++j) sl += part[j];
     return halves_sum(sl);
+  };
+  // Sum of a row's PW wave partials, read as 16-byte vectors and added in
+  // wave order.
+  auto row_total = [&](const float* pr) {
+    float S = 0.f;
+#pragma unroll
+    for (int w = 0; w < PW; w += 4) {
+      const f32x4v v = *reinterpret_cast<const f32x4v*>(pr + w);
+      S += v[0];
+      S += v[1];
+      S += v[2];
+      S += v[3];
+    }
+    return S;
   };
   auto row_kslot = [&](int blk) {
     const u32x4 k = L.rks[blk * 32 + l32];
@@ -556,15 +574,13 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
       nbh = L.rop[nb][0][lane];
       nbl = L.rop[nb][1][lane];
       nbk = row_kslot(nb);
+      const int r = blk * 32 + l32;
+      const float ar = L.af[r];  // read before the barrier, off the wait chain
       float sl = sweep_block(bh, bl, bk);
       const int buf = blk & 1;
-      if (h == 0) L.rowpart[buf][wave][l32] = sl;
+      if (h == 0) L.rowpart[buf][l32][wave] = sl;
       __syncthreads();
-      float S = 0.f;
-#pragma unroll
-      for (int w = 0; w < PW; ++w) S += L.rowpart[buf][w][l32];
-      const int r = blk * 32 + l32;
-      const float ar = L.af[r];
+      float S = row_total(L.rowpart[buf][l32]);
       const bool bad = ar > 0.f && !in_safe_range((double)S);
       if (__ballot(bad)) {
         // Re-centre the rows that left the range on their maximum exponent.
@@ -588,11 +604,9 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
         }
         tiles(blk, bk);
         sl = exp_rowsum();
-        if (h == 0) L.rowpart[buf][wave][l32] = sl;
+        if (h == 0) L.rowpart[buf][l32][wave] = sl;
         __syncthreads();
-        S = 0.f;
-#pragma unroll
-        for (int w = 0; w < PW; ++w) S += L.rowpart[buf][w][l32];
+        S = row_total(L.rowpart[buf][l32]);
         if (wave == 0 && h == 0 && bad) {
           L.c2s[r] = c2;  // the shift actually used
           L.rks[r] = bk;
