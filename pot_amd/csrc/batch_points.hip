@@ -328,17 +328,34 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     for (int j = 1; j < NS; ++j) sl += part[j];
     return halves_sum(sl);
   };
-  // tiles_ops + exp_rowsum of one row block, scheduled so the matrix core and
-  // the VALU work at the same time: the MFMA chains of the first two tiles,
-  // then the chains of the other tiles each followed by a share of the exp2
-  // and row-sum work of the first two, then the exp2 work of the others.
-  // (Both waves of a SIMD meet the block barrier in step, so the other wave
-  // cannot fill one unit while this one works on the other.)  Same
-  // arithmetic, in the same order, as tiles_ops followed by exp_rowsum.
+  // phase_a + sweep_block compute one row block as tiles_ops followed by
+  // exp_rowsum (same arithmetic, same order), scheduled so the matrix core
+  // and the VALU work at the same time.  (Both waves of a SIMD meet the block
+  // barrier in step, so the other wave cannot fill one unit while this one
+  // works on the other.)
+  constexpr int NA = NS < 2 ? NS : 2;  // tiles whose chains open a block
+  constexpr int NB = NS - NA;          // tiles whose chains overlap exp2 work
+  constexpr int NP = 1;                // opening chains issued a block early
+  // The MFMA chain of tile 0 of a row block into dst.  The main sweep issues
+  // it for the next block before the block barrier, so the matrix core works
+  // while the waves meet and exchange row sums (one tile: more live tiles
+  // across the barrier spill).
+  auto phase_a = [&](u32x4 bh, u32x4 bl, u32x4 bk, f32x16* dst) {
+    const f32x16 z = {};
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      dst[j] = mfma_h(alo[j], bh, z);
+      dst[j] = mfma_h(ahi[j], bl, dst[j]);
+      dst[j] = mfma_h(ahi[j], bh, dst[j]);
+      dst[j] = mfma_b(ak[j], bk, dst[j]);
+    }
+  };
+  // The rest of a row block, with e[0 .. NP-1] already holding their chains:
+  // the other opening chains, then the remaining chains each followed by a
+  // share of the exp2 / row-sum work of the first tiles, then the exp2 work
+  // of the others.
   auto sweep_block = [&](u32x4 bh, u32x4 bl, u32x4 bk) {
     const f32x16 z = {};
-    constexpr int NA = NS < 2 ? NS : 2;  // tiles whose chains open the block
-    constexpr int NB = NS - NA;          // tiles whose chains overlap exp2 work
     auto mf = [&](int j, int k) {
       if (k == 0) e[j] = mfma_h(alo[j], bh, z);
       if (k == 1) e[j] = mfma_h(ahi[j], bl, e[j]);
@@ -355,8 +372,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int j = 0; j < NA; ++j) mf(j, k);
-    __builtin_amdgcn_sched_group_barrier(0x008, 4 * NA, 0);
+      for (int j = NP; j < NA; ++j) mf(j, k);
     if constexpr (NB > 0) {
       constexpr int PER = 16 * NA / (4 * NB);  // exp2 elements per MFMA
 #pragma unroll
@@ -565,6 +581,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     // The row operands of the next block are read one block ahead, so their
     // LDS latency overlaps this block's work.
     u32x4 nbh = L.rop[0][0][lane], nbl = L.rop[0][1][lane], nbk = row_kslot(0);
+    phase_a(nbh, nbl, nbk, e);
     for (int blk = 0; blk < nblk; ++blk) {
       u32x4 bk = nbk;
       const u32x4 bh = nbh, bl = nbl;
@@ -579,6 +596,11 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
       float sl = sweep_block(bh, bl, bk);
       const int buf = blk & 1;
       if (h == 0) L.rowpart[buf][l32][wave] = sl;
+      // The next block's tile-0 chain (after the last block, this block's
+      // again, unused).
+      f32x16 na[NP];
+      phase_a(nbh, nbl, nbk, na);
+      __builtin_amdgcn_sched_barrier(0);  // keep the chain ahead of the barrier
       __syncthreads();
       float S = row_total(L.rowpart[buf][l32]);
       const bool bad = ar > 0.f && !in_safe_range((double)S);
@@ -618,6 +640,8 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[j][i] = __builtin_fmaf(e[j][i], wr, acc[j][i]);
       if (wave == 0 && h == 0) L.sh[r] = S;
+#pragma unroll
+      for (int j = 0; j < NP; ++j) e[j] = na[j];
     }
     columns_out(std::integral_constant<bool, false>(), L.colx);
     __syncthreads();
