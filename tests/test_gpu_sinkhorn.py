@@ -201,3 +201,17 @@ def test_dist_matches_bruteforce():
     np.testing.assert_allclose(D, brute, atol=1e-12)
     Dxx = pot_amd.dist(x)
     assert np.all(np.diag(Dxx) == 0)
+
+
+def test_reference_cpu_case_fixed_iterations_match_oracle():
+    # the reference's CPU-runnable configuration (n = m = 2048, float64, M / max,
+    # reg 1e-2) at 30 fixed iterations: exact niter, plan to 1e-10 relative
+    from pot_amd import problems
+    a, b, M = problems.config1()
+    G, lg = pot_amd.sinkhorn(a, b, M, 1e-2, method="sinkhorn_log", stopThr=0.0, numItermax=30,
+                             log=True, warn=False)
+    print("gpu done", flush=True)
+    Go, lo = osk.sinkhorn_log(a, b, M, 1e-2, numItermax=30, stopThr=0.0, log=True, warn=False)
+    assert lg["niter"] == lo["niter"]
+    assert _rel_plan_err(G, Go) < 1e-10
+    np.testing.assert_allclose(lg["err"], lo["err"], rtol=1e-9)
