@@ -61,6 +61,8 @@ def _args():
     ap.add_argument("--cost", default="points", choices=["dense", "points"],
                     help="warmstart / ragged: cost computed on the fly from the points "
                          "(points) or materialised in HBM (dense)")
+    ap.add_argument("--split", type=int, default=1,
+                    help="ragged: share large problems among workgroups when the batch is small")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cold-only", action="store_true")
     return ap.parse_args()
@@ -339,7 +341,8 @@ def run_ragged(args, torch, dev):
     Yp = torch.cat(Y)
     G = ragged.RaggedGraph(ns, ms, 16, numItermax=2000, stopThr=1e-6, check_every=10,
                            waves=args.waves, prefetch=args.prefetch,
-                           occupancy=args.occ, cost=args.cost) if args.graph else None
+                           occupancy=args.occ, cost=args.cost,
+                           split="auto" if args.split else None) if args.graph else None
     times, kms, abytes = [], [], []
     res = None
     for k in range(args.warmup + args.steps):

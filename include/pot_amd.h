@@ -209,6 +209,37 @@ int potamd_sinkhorn_log_batch_points(int batch, int max_n, int max_m, int d, con
                                      int* status, double* err_last, double* err_first,
                                      double* cmax, int waves, int* queue, void* stream);
 
+/* The same batch with large problems split over several workgroups, for
+ * batches too small to fill the GPU with one workgroup per problem (the
+ * slowest problem then sets the time).  No counterpart in the reference
+ * (ot/bregman/_sinkhorn.py:880-906 per problem, as above).
+ * items, order: the queue, `items` entries, each p | k << 24 (problem p, part
+ *    k); the parts of a split problem must be taken together, so list them
+ *    next to each other.
+ * split: device int[B], 0 for a whole problem, else parts | slot << 4 with
+ *    parts 2..4 (each part needs at least one 32-row block: n >= 32 parts)
+ *    and slot < nsplit, one slot per split problem.
+ * split_ws: nsplit exchange slots (potamd_points_split_workspace_size bytes,
+ *    256-byte aligned), zero before the first launch; the kernel leaves the
+ *    counters zero again, like the queue.  The parts combine their column
+ *    reductions there in part order, so every part takes the same decisions;
+ *    a part that waits too long for the others gives up and the problem
+ *    reports status 2.
+ * queue: required; 8 wavefronts per part.  Other arguments as above.
+ */
+size_t potamd_points_split_workspace_size(int nsplit);
+int potamd_sinkhorn_log_batch_points_split(int batch, int max_n, int max_m, int d, const float* X,
+                                           const float* Y, const int64_t* xoff,
+                                           const int64_t* yoff, const int* ns, const int* ms,
+                                           const double* regs, int normalize, const double* a,
+                                           const double* b, double* u, double* v, int max_iter,
+                                           int check_every, double stop_thr, double fallback_thr,
+                                           int items, const int* order, const int* split,
+                                           int nsplit, void* split_ws, size_t split_ws_bytes,
+                                           int* niter, int* status, double* err_last,
+                                           double* err_first, double* cmax, int* queue,
+                                           void* stream);
+
 /* Per-problem maximum of |x_r - y_c|^2 (the M.max() of ot.dist(x, y) /
  * M.max()), the same device search as potamd_sinkhorn_log_batch_points with
  * normalize = 1 (MFMA pass, then the best pair of each lane evaluated in fp64

@@ -85,7 +85,20 @@ struct PointsArgs {
   double* cmax;          // [B] the cost maximum used (out; in when normalize == 2); may be null
   int batch;             // B
   int* queue;            // [2] work-queue counters (zero at launch, zero at exit) or null
+  int items;             // entries of order: B, or more when problems are split
+  const int* split;      // [B] 0 (whole problem) or parts | slot << 4; may be null
+  char* split_ws;        // exchange slots of the split problems (kSplitSlot bytes each)
+  int nsplit;            // number of slots
 };
+
+// Split problems: the parts of problem p (2 or 4 workgroups, each owning a
+// contiguous range of row blocks) combine their column reductions through
+// one exchange slot: a counter, then double-buffered per-part columns.
+constexpr int kSplitMax = 4;
+constexpr size_t kSplitCols = 256;                                    // byte offset
+constexpr size_t kSplitDbl = kSplitCols + 2ull * kSplitMax * kPMax * 4;  // byte offset
+constexpr size_t kSplitSlot = kSplitDbl + 256;
+constexpr int kSplitSpin = 1 << 22;  // polls (with a sleep each) before giving up
 
 __device__ __forceinline__ unsigned bf16_bits(float f) {
   unsigned u = __float_as_uint(f);
@@ -241,9 +254,9 @@ struct PointsLds {
 // Everything after the centring, for problems whose column tiles take NS
 // slots per wave (NS = ceil(ceil(m / 32) / 8)); slots past m hold padding
 // columns whose potential is -inf, so they add nothing.
-template <int PW, int NS>
+template <int PW, int NS, bool SPLIT>
 __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>& L, int p, int n,
-                                             int m, double ext) {
+                                             int m, double ext, int part, int K, char* sws) {
   constexpr int PT = PW * kWave;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int h = lane >> 5, l32 = lane & 31;
@@ -256,6 +269,88 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   double* up = A.u + xo;
   double* vp = A.v + yo;
   const int nblk = (n + 31) / 32;
+  // Row blocks of this workgroup: all of them, or part `part` of K.
+  const int blk0 = SPLIT ? part * nblk / K : 0;
+  const int blk1 = SPLIT ? (part + 1) * nblk / K : nblk;
+  // Split problems: wait until the K parts reach the same exchange (a
+  // counter in the slot; false after a bounded wait), and combine a
+  // per-column partial (max or sum, parts added in part order, so every part
+  // gets the same values) or a double maximum.
+  // The exchanged values are written and read with device-scope atomic
+  // accesses (which bypass the caches that are not coherent across the
+  // GPU), so the counter needs no cache write-back or invalidation around
+  // it: each thread waits for its own stores to complete before the
+  // workgroup barrier, and the counter is incremented after that barrier.
+  int sgen = 0;
+  auto group_sync = [&]() -> bool {
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's exchange stores have completed
+    __syncthreads();
+    if (tid == 0) {
+      int* cnt = reinterpret_cast<int*>(sws);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int target = K * (sgen + 1);
+      int ok = 1;
+      for (int spin = 0;
+           __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++spin) {
+        if (spin >= kSplitSpin) {
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      L.flag = ok;
+    }
+    __syncthreads();
+    ++sgen;
+    return uniform_i(L.flag) != 0;
+  };
+  auto put = [](float* q, float x) {
+    __hip_atomic_store(q, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  auto get = [](const float* q) {
+    return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  auto exchange_cols = [&](float* col, bool is_max) -> bool {
+    float* xf = reinterpret_cast<float*>(sws + kSplitCols);
+    const int buf = sgen & 1;
+    for (int c = tid; c < m; c += PT) put(&xf[(buf * kSplitMax + part) * kPMax + c], col[c]);
+    if (!group_sync()) return false;
+    for (int c = tid; c < m; c += PT) {
+      float v = get(&xf[(buf * kSplitMax) * kPMax + c]);
+      for (int k = 1; k < K; ++k) {
+        const float x = get(&xf[(buf * kSplitMax + k) * kPMax + c]);
+        v = is_max ? fmaxf(v, x) : v + x;
+      }
+      col[c] = v;
+    }
+    __syncthreads();
+    return true;
+  };
+  auto exchange_max = [&](double& x) -> bool {
+    double* xd = reinterpret_cast<double*>(sws + kSplitDbl);
+    const int buf = sgen & 1;
+    if (tid == 0)
+      __hip_atomic_store(&xd[buf * kSplitMax + part], x, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    if (!group_sync()) return false;
+    double v = __hip_atomic_load(&xd[buf * kSplitMax], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 1; k < K; ++k)
+      v = fmax(v, __hip_atomic_load(&xd[buf * kSplitMax + k], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT));
+    x = uniform_d(v);
+    __syncthreads();
+    return true;
+  };
+  // A part that timed out reports the problem as not solved.
+  auto abandon = [&]() {
+    if (tid == 0 && part == 0) {
+      A.niter[p] = 0;
+      A.status[p] = kNumerical;
+    }
+  };
   // Tile j of this wave covers columns 32 * (wave + 8 j) + [0, 32).
   auto col_of = [&](int j) { return 32 * (wave + PW * j) + l32; };
   // Column of value i of tile j in the MFMA output layout.
@@ -451,7 +546,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     __syncthreads();
     float best = __builtin_inff();
     int code = -1;
-    for (int blk = 0; blk < nblk; ++blk) {
+    for (int blk = blk0; blk < blk1; ++blk) {
       tiles(blk, row_kslot(blk));
 #pragma unroll
       for (int j = 0; j < NS; ++j) {
@@ -481,6 +576,9 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     for (int w = 0; w < PW; ++w) cmax = fmax(cmax, L.red[w]);
     cmax = uniform_d(cmax);
     __syncthreads();
+    if constexpr (SPLIT) {
+      if (!exchange_max(cmax)) return abandon();
+    }
     if (A.max_only) {
       if (tid == 0) A.cmax[p] = cmax;
       return;
@@ -529,20 +627,23 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   for (int j = 0; j < NS; ++j)
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[j][i] = -__builtin_inff();
-  for (int blk = 0; blk < nblk; ++blk) {
+  for (int blk = blk0; blk < blk1; ++blk) {
     tiles(blk, row_kslot(blk));
 #pragma unroll
     for (int j = 0; j < NS; ++j) acc[j] = __builtin_elementwise_max(acc[j], e[j]);
   }
   columns_out(std::integral_constant<bool, true>(), L.sh);
   __syncthreads();
+  if constexpr (SPLIT) {
+    if (!exchange_cols(L.sh, true)) return abandon();
+  }
   set_col_kslots([&](int c) {
     const float g = L.sh[c < m ? c : 0];
     return (c < m && __builtin_isfinite(g)) ? -L.yn[c] - (double)g : -__builtin_inf();
   });
 #pragma unroll
   for (int j = 0; j < NS; ++j) acc[j] = f32x16{};
-  for (int blk = 0; blk < nblk; ++blk) {
+  for (int blk = blk0; blk < blk1; ++blk) {
     tiles(blk, row_kslot(blk));
 #pragma unroll
     for (int j = 0; j < NS; ++j)
@@ -551,6 +652,9 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   }
   columns_out(std::integral_constant<bool, false>(), L.colx);
   __syncthreads();
+  if constexpr (SPLIT) {
+    if (!exchange_cols(L.colx, false)) return abandon();
+  }
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
     const int c = col_of(j);
@@ -580,14 +684,14 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     for (int j = 0; j < NS; ++j) acc[j] = f32x16{};
     // The row operands of the next block are read one block ahead, so their
     // LDS latency overlaps this block's work.
-    u32x4 nbh = L.rop[0][0][lane], nbl = L.rop[0][1][lane], nbk = row_kslot(0);
+    u32x4 nbh = L.rop[blk0][0][lane], nbl = L.rop[blk0][1][lane], nbk = row_kslot(blk0);
     phase_a(nbh, nbl, nbk, e);
-    for (int blk = 0; blk < nblk; ++blk) {
+    for (int blk = blk0; blk < blk1; ++blk) {
       u32x4 bk = nbk;
       const u32x4 bh = nbh, bl = nbl;
       // Unconditional (the last block re-reads itself), so the block's MFMA
       // and exp2 work stay in one scheduling region.
-      const int nb = blk + 1 < nblk ? blk + 1 : blk;
+      const int nb = blk + 1 < blk1 ? blk + 1 : blk;
       nbh = L.rop[nb][0][lane];
       nbl = L.rop[nb][1][lane];
       nbk = row_kslot(nb);
@@ -645,9 +749,13 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     }
     columns_out(std::integral_constant<bool, false>(), L.colx);
     __syncthreads();
+    if constexpr (SPLIT) {
+      if (!exchange_cols(L.colx, false)) return abandon();
+    }
 
     // ---- row potentials, marginal error, stop decision ----
-    for (int r = tid; r < n; r += PT) {
+    const int rend = SPLIT ? (blk1 * 32 < n ? blk1 * 32 : n) : n;
+    for (int r = blk0 * 32 + tid; r < rend; r += PT) {
       if (L.af[r] > 0.f) {
         // sh[r] lies in [2^-60, 2^60] (the safe range of the sweep).
         const double c = L.c2s[r] + (double)__builtin_amdgcn_logf(L.sh[r]);
@@ -718,10 +826,15 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   if (!again) break;
   }
 
-  // ---- outputs (natural-log potentials) ----
+  // ---- outputs (natural-log potentials; a split problem's parts write their
+  // own rows, part 0 the columns and the status) ----
   __syncthreads();
-  for (int i = tid; i < n; i += PT)
-    up[i] = L.af[i] > 0.f ? (log2(ap[i]) - L.c2s[i]) * kLn2 : -__builtin_inf();
+  {
+    const int rend = SPLIT ? (blk1 * 32 < n ? blk1 * 32 : n) : n;
+    for (int i = blk0 * 32 + tid; i < rend; i += PT)
+      up[i] = L.af[i] > 0.f ? (log2(ap[i]) - L.c2s[i]) * kLn2 : -__builtin_inf();
+  }
+  if (part != 0) return;
   for (int j = tid; j < m; j += PT) vp[j] = L.vsh[j] * kLn2;
   if (tid == 0) {
     A.niter[p] = niter;
@@ -737,7 +850,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
 // declared here, so that the persistent kernel's loop around it is a plain
 // uniform loop.
 template <int PW>
-__device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int p) {
+__device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int item) {
   __shared__ PointsLds<PW> L;
   // Arguments of an out-of-line function count as per-lane values for the
   // compiler: the ones that steer barriers are made visibly uniform.
@@ -745,7 +858,21 @@ __device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int 
   A.normalize = uniform_i(Ain.normalize);
   A.max_only = uniform_i(Ain.max_only);
   A.max_iter = uniform_i(Ain.max_iter);
-  p = uniform_i(p);
+  item = uniform_i(item);
+  // An item is problem p, part `part` of a split problem (part 0 otherwise).
+  const int p = item & 0xFFFFFF;
+  const int part = item >> 24;
+  const int sp = A.split ? uniform_i(A.split[p]) : 0;
+  const int K = sp ? (sp & 15) : 1;
+  if (p >= A.batch || part >= K || (sp && (K < 2 || K > kSplitMax || (sp >> 4) >= A.nsplit))) {
+    // A malformed item or split entry: nothing is touched but the status.
+    if (p < A.batch && threadIdx.x == 0 && !A.max_only) {
+      A.niter[p] = 0;
+      A.status[p] = kNumerical;
+    }
+    return;
+  }
+  char* sws = sp ? A.split_ws + (size_t)(sp >> 4) * kSplitSlot : nullptr;
   constexpr int PT = PW * kWave;
   constexpr int NG = PT / 16;  // point groups of the centring reduction
   const int tid = threadIdx.x;
@@ -811,20 +938,30 @@ __device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int 
     __syncthreads();  // the scratch is the row-operand area, packed next
   }
   const int ns = ((m + 31) / 32 + PW - 1) / PW;
+  if (K > 1) {
+    // Split problems (8 waves only): the same solve over the part's rows.
+    if constexpr (PW == 8) {
+      if (ns <= 2)
+        solve_points<PW, 2, true>(A, L, p, n, m, ext, part, K, sws);
+      else
+        solve_points<PW, 4, true>(A, L, p, n, m, ext, part, K, sws);
+    }
+    return;
+  }
   if constexpr (PW == 16) {
     if (ns <= 1)
-      solve_points<PW, 1>(A, L, p, n, m, ext);
+      solve_points<PW, 1, false>(A, L, p, n, m, ext, 0, 1, nullptr);
     else
-      solve_points<PW, 2>(A, L, p, n, m, ext);
+      solve_points<PW, 2, false>(A, L, p, n, m, ext, 0, 1, nullptr);
   } else {
     if (ns <= 1)
-      solve_points<PW, 1>(A, L, p, n, m, ext);
+      solve_points<PW, 1, false>(A, L, p, n, m, ext, 0, 1, nullptr);
     else if (ns == 2)
-      solve_points<PW, 2>(A, L, p, n, m, ext);
+      solve_points<PW, 2, false>(A, L, p, n, m, ext, 0, 1, nullptr);
     else if (ns == 3)
-      solve_points<PW, 3>(A, L, p, n, m, ext);
+      solve_points<PW, 3, false>(A, L, p, n, m, ext, 0, 1, nullptr);
     else
-      solve_points<PW, 4>(A, L, p, n, m, ext);
+      solve_points<PW, 4, false>(A, L, p, n, m, ext, 0, 1, nullptr);
   }
 }
 
@@ -847,12 +984,16 @@ __global__ __launch_bounds__(PW * kWave) void batch_points_kernel(PointsArgs A) 
     __syncthreads();
     const int q = uniform_i(qslot);
     __syncthreads();  // everyone has the slot before it is taken again
-    if (q < 0 || q >= A.batch) break;
+    if (q < 0 || q >= A.items) break;
     one_problem<PW>(A, A.order ? uniform_i(A.order[q]) : q);
   }
   if (threadIdx.x == 0) {
     __threadfence();
     if (atomicAdd(&A.queue[1], 1) == (int)gridDim.x - 1) {
+      // Every part has left its problem: the split counters go back to zero
+      // with the queue.
+      for (int k = 0; k < A.nsplit; ++k)
+        atomicExch(reinterpret_cast<int*>(A.split_ws + (size_t)k * kSplitSlot), 0);
       atomicExch(&A.queue[0], 0);
       atomicExch(&A.queue[1], 0);
     }
@@ -864,12 +1005,14 @@ __global__ __launch_bounds__(PW * kWave) void batch_points_kernel(PointsArgs A) 
 
 using namespace potamd;
 
-extern "C" int potamd_sinkhorn_log_batch_points(
-    int batch, int max_n, int max_m, int d, const float* X, const float* Y, const int64_t* xoff,
-    const int64_t* yoff, const int* ns, const int* ms, const double* regs, int normalize,
-    const double* a, const double* b, double* u, double* v, int max_iter, int check_every,
-    double stop_thr, double fallback_thr, const int* order, int* niter, int* status,
-    double* err_last, double* err_first, double* cmax, int waves, int* queue, void* stream) {
+namespace {
+int points_launch(int batch, int max_n, int max_m, int d, const float* X, const float* Y,
+                  const int64_t* xoff, const int64_t* yoff, const int* ns, const int* ms,
+                  const double* regs, int normalize, const double* a, const double* b, double* u,
+                  double* v, int max_iter, int check_every, double stop_thr,
+                  double fallback_thr, const int* order, int* niter, int* status,
+                  double* err_last, double* err_first, double* cmax, int waves, int* queue,
+                  int items, const int* split, char* split_ws, int nsplit, void* stream) {
   if (batch < 1 || max_n < 1 || max_m < 1 || d < 1 || !X || !Y || !xoff || !yoff || !ns ||
       !ms || !regs || !a || !b || !u || !v || max_iter < 1 || !niter || !status ||
       check_every < 0 || normalize < 0 || normalize > 2 || (normalize == 2 && !cmax))
@@ -905,9 +1048,13 @@ extern "C" int potamd_sinkhorn_log_batch_points(
   pa.cmax = cmax;
   pa.batch = batch;
   pa.queue = queue;
+  pa.items = items;
+  pa.split = split;
+  pa.split_ws = split_ws;
+  pa.nsplit = nsplit;
   // Persistent launch with the queue: one workgroup per CU (the LDS of one
   // problem fills most of a CU).
-  const int grid = queue ? (batch < device_cus() ? batch : device_cus()) : batch;
+  const int grid = queue ? (items < device_cus() ? items : device_cus()) : items;
   if (waves == 16)
     hipLaunchKernelGGL(batch_points_kernel<16>, dim3(grid), dim3(16 * kWave), 0,
                        static_cast<hipStream_t>(stream), pa);
@@ -916,6 +1063,41 @@ extern "C" int potamd_sinkhorn_log_batch_points(
                        static_cast<hipStream_t>(stream), pa);
   HIP_TRY(hipGetLastError());
   return 0;
+}
+}  // namespace
+
+extern "C" int potamd_sinkhorn_log_batch_points(
+    int batch, int max_n, int max_m, int d, const float* X, const float* Y, const int64_t* xoff,
+    const int64_t* yoff, const int* ns, const int* ms, const double* regs, int normalize,
+    const double* a, const double* b, double* u, double* v, int max_iter, int check_every,
+    double stop_thr, double fallback_thr, const int* order, int* niter, int* status,
+    double* err_last, double* err_first, double* cmax, int waves, int* queue, void* stream) {
+  return points_launch(batch, max_n, max_m, d, X, Y, xoff, yoff, ns, ms, regs, normalize, a, b,
+                       u, v, max_iter, check_every, stop_thr, fallback_thr, order, niter, status,
+                       err_last, err_first, cmax, waves, queue, batch, nullptr, nullptr, 0,
+                       stream);
+}
+
+extern "C" size_t potamd_points_split_workspace_size(int nsplit) {
+  return nsplit > 0 ? (size_t)nsplit * kSplitSlot : 0;
+}
+
+extern "C" int potamd_sinkhorn_log_batch_points_split(
+    int batch, int max_n, int max_m, int d, const float* X, const float* Y, const int64_t* xoff,
+    const int64_t* yoff, const int* ns, const int* ms, const double* regs, int normalize,
+    const double* a, const double* b, double* u, double* v, int max_iter, int check_every,
+    double stop_thr, double fallback_thr, int items, const int* order, const int* split,
+    int nsplit, void* split_ws, size_t split_ws_bytes, int* niter, int* status,
+    double* err_last, double* err_first, double* cmax, int* queue, void* stream) {
+  if (items < batch || !order || !split || !queue || nsplit < 1 || !split_ws ||
+      split_ws_bytes < potamd_points_split_workspace_size(nsplit) ||
+      (reinterpret_cast<uintptr_t>(split_ws) & 255) != 0)
+    return POTAMD_EINVAL;
+  if (batch >= (1 << 24) || items > (1 << 30)) return POTAMD_EUNSUPPORTED;
+  return points_launch(batch, max_n, max_m, d, X, Y, xoff, yoff, ns, ms, regs, normalize, a, b,
+                       u, v, max_iter, check_every, stop_thr, fallback_thr, order, niter, status,
+                       err_last, err_first, cmax, 8, queue, items, split,
+                       static_cast<char*>(split_ws), nsplit, stream);
 }
 
 extern "C" int potamd_ragged_points_cmax(int batch, int max_n, int max_m, int d, const float* X,
@@ -939,6 +1121,7 @@ extern "C" int potamd_ragged_points_cmax(int batch, int max_n, int max_m, int d,
   pa.max_only = 1;
   pa.cmax = cmax;
   pa.batch = batch;
+  pa.items = batch;
   hipLaunchKernelGGL(batch_points_kernel<kPWDefault>, dim3(batch), dim3(kPWDefault * kWave), 0,
                      static_cast<hipStream_t>(stream), pa);
   HIP_TRY(hipGetLastError());

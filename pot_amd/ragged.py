@@ -323,6 +323,52 @@ def _points_order(ns, ms, regs):
     return np.argsort(-work, kind="stable").astype(np.int32)
 
 
+def split_plan(ns, ms, cus=None):
+    """Workgroups per problem for a batch too small to fill the GPU with one
+    workgroup per problem (at most 2 problems per CU): 4 for problems of at
+    least 3 * 2^17 elements, 2 from 2^17, else 1 (each part needs 32 rows).
+    Larger batches are not split (their work already fills the GPU)."""
+    ns = np.asarray(ns, dtype=np.int64)
+    ms = np.asarray(ms, dtype=np.int64)
+    parts = np.ones(len(ns), dtype=np.int32)
+    if cus is None:
+        cus = torch.cuda.get_device_properties(_device.require_gpu()).multi_processor_count
+    if len(ns) > 2 * cus:
+        return parts
+    nm = ns * ms
+    parts[(nm >= 1 << 17) & (ns >= 64)] = 2
+    parts[(nm >= 3 << 17) & (ns >= 128)] = 4
+    return parts
+
+
+def _split_tables(parts):
+    """split[p] = parts | slot << 4 for the split problems (0 otherwise) and
+    the number of slots."""
+    split = np.zeros(len(parts), dtype=np.int32)
+    big = np.nonzero(parts > 1)[0]
+    split[big] = parts[big] | (np.arange(len(big), dtype=np.int32) << 4)
+    return split, len(big)
+
+
+def _split_items(order, parts):
+    """Queue items: the problems in launch order, each as its parts in a row."""
+    order = np.asarray(order, dtype=np.int64)
+    reps = parts[order]
+    probs = np.repeat(order, reps)
+    first = np.repeat(np.cumsum(reps) - reps, reps)
+    part = np.arange(len(probs)) - first
+    return (probs | (part << 24)).astype(np.int32)
+
+
+def _check_parts(parts, ns):
+    parts = np.asarray(parts, dtype=np.int32).reshape(-1)
+    if len(parts) != len(ns) or parts.min() < 1 or parts.max() > 4:
+        raise ValueError("split: one part count in 1..4 per problem")
+    if np.any(np.asarray(ns) < 32 * parts):
+        raise ValueError("split: each part needs at least 32 rows (n >= 32 * parts)")
+    return parts
+
+
 def _launch_points(lib, P, X, Y, regs, u, v, niter, status, e_last, e_first, cmax, order,
                    numItermax, check_every, stopThr, fallback_thr, stream, cmax_given=False,
                    waves=0, queue=None):
@@ -336,8 +382,31 @@ def _launch_points(lib, P, X, Y, regs, u, v, niter, status, e_last, e_first, cma
     _lib.check(rc, "potamd_sinkhorn_log_batch_points")
 
 
+def _launch_points_split(lib, P, X, Y, regs, u, v, niter, status, e_last, e_first, cmax, items,
+                         split, nsplit, sws, numItermax, check_every, stopThr, fallback_thr,
+                         stream, queue, cmax_given=False):
+    mode = (2 if cmax_given else 1) if P.normalize else 0
+    rc = lib.potamd_sinkhorn_log_batch_points_split(
+        P.batch, int(P.ns.max()), int(P.ms.max()), P.d, _ptr(X), _ptr(Y), _ptr(P.aoff),
+        _ptr(P.boff), _ptr(P.ns_d), _ptr(P.ms_d), _ptr(regs), mode, _ptr(P.a),
+        _ptr(P.b), _ptr(u), _ptr(v), int(numItermax), int(check_every), float(stopThr),
+        float(fallback_thr), int(items.numel()), _ptr(items), _ptr(split), int(nsplit),
+        _ptr(sws), sws.numel(), _ptr(niter), _ptr(status), _ptr(e_last), _ptr(e_first),
+        _ptr(cmax), _ptr(queue), stream)
+    _lib.check(rc, "potamd_sinkhorn_log_batch_points_split")
+
+
+def _split_workspace(lib, nsplit, dev):
+    """Zeroed exchange slots, 256-byte aligned (the kernel leaves them zero)."""
+    nbytes = int(lib.potamd_points_split_workspace_size(int(nsplit)))
+    buf = torch.zeros(nbytes + 256, dtype=torch.uint8, device=dev)
+    off = (-buf.data_ptr()) % 256
+    return buf[off:off + nbytes]
+
+
 def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=None,
-                               fallback_thr=0.0, check_every=10, cmax=None, waves=0):
+                               fallback_thr=0.0, check_every=10, cmax=None, waves=0,
+                               split="auto"):
     """``sinkhorn_log_ragged`` on point clouds, the cost computed on the fly.
 
     P: a RaggedPoints.  Same loop, stop rule, warm start and fallback as
@@ -346,7 +415,10 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
     sweep and never stored.  ``cmax``: the maxima from ``points_cmax`` (the
     solve then skips its own search).  ``waves``: wavefronts per problem (8
     or 16; 0 = the autotuned value of this batch class, else the library
-    default).  Returns a RaggedResult whose ``cmax``
+    default).  ``split``: "auto" (``split_plan``: only batches too small to
+    fill the GPU are split), None (one workgroup per problem) or workgroups
+    per problem (1..4): a split problem's rows are shared by several
+    workgroups (8 waves each).  Returns a RaggedResult whose ``cmax``
     holds the cost maxima; asynchronous on the current stream.
     """
     lib = _lib.load()
@@ -370,6 +442,20 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
     waves = _tuned("points", P.ns, P.ms, P.d, waves)[0]
     # work-queue counters of the persistent launch (the kernel leaves them zero)
     queue = torch.zeros(2, dtype=torch.int32, device=dev)
+    parts = None
+    if split is not None:
+        parts = split_plan(P.ns, P.ms) if (isinstance(split, str) and split == "auto") \
+            else _check_parts(split, P.ns)
+    if parts is not None and (parts > 1).any():
+        table, nsplit = _split_tables(parts)
+        items = torch.from_numpy(_split_items(order.cpu().numpy(), parts)).to(dev)
+        sws = _split_workspace(lib, nsplit, dev)
+        _launch_points_split(lib, P, P.X, P.Y, P.regs, u, v, niter, status, e_last, e_first,
+                             cmax, items, torch.from_numpy(table).to(dev), nsplit, sws,
+                             numItermax, check_every, stopThr, fallback_thr,
+                             ctypes.c_void_p(_device.stream_ptr(dev)), queue, cmax_given=given)
+        return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
+                            err_last=e_last, cmax=cmax)
     _launch_points(lib, P, P.X, P.Y, P.regs, u, v, niter, status, e_last, e_first, cmax, order,
                    numItermax, check_every, stopThr, fallback_thr,
                    ctypes.c_void_p(_device.stream_ptr(dev)), cmax_given=given, waves=waves,
@@ -395,7 +481,7 @@ class RaggedGraph:
 
     def __init__(self, ns, ms, d, numItermax=1000, stopThr=1e-9, check_every=10,
                  fallback_thr=0.0, waves=0, prefetch=0, occupancy=0, device=None,
-                 cost="dense"):
+                 cost="dense", split="auto"):
         if cost not in ("dense", "points"):
             raise ValueError("cost must be 'dense' or 'points'")
         self.cost = cost
@@ -428,6 +514,19 @@ class RaggedGraph:
         self.e_last = torch.zeros(B, dtype=torch.float64, device=dev)
         self.order = torch.arange(B, dtype=torch.int32, device=dev)
         self.queue = torch.zeros(2, dtype=torch.int32, device=dev)  # left zero by each launch
+        # Split problems (cost="points"): parts per problem fixed by the sizes,
+        # the queue items re-ordered on each solve, the exchange slots zero.
+        self.parts = None
+        if split is not None and cost == "points":
+            parts = split_plan(self.ns, self.ms) if (isinstance(split, str) and split == "auto") \
+                else _check_parts(split, self.ns)
+            if (parts > 1).any():
+                self.parts = parts
+                table, self.nsplit = _split_tables(parts)
+                self.split = torch.from_numpy(table).to(dev)
+                self.sws = _split_workspace(self.lib, self.nsplit, dev)
+                self.items = torch.from_numpy(
+                    _split_items(np.arange(B), parts)).to(dev)
         waves, prefetch, occupancy = _tuned(cost, self.ns, self.ms,
                                             self.d if cost == "points" else 0,
                                             waves, prefetch, occupancy)
@@ -455,6 +554,13 @@ class RaggedGraph:
         if self.cost == "points":
             self.u.copy_(self.u0)
             max_iter, check_every, stop, fb, wv = self.settings
+            if self.parts is not None:
+                _launch_points_split(self.lib, self, self.X, self.Y, self.reg_in, self.u,
+                                     self.v, self.niter, self.status, self.e_last,
+                                     self.e_first, self.cmax, self.items, self.split,
+                                     self.nsplit, self.sws, max_iter, check_every, stop, fb,
+                                     stream, self.queue)
+                return
             _launch_points(self.lib, self, self.X, self.Y, self.reg_in, self.u, self.v,
                            self.niter, self.status, self.e_last, self.e_first, self.cmax,
                            self.order, max_iter, check_every, stop, fb, stream,
@@ -493,6 +599,8 @@ class RaggedGraph:
         else:
             order = _check_order(order, len(self.ns))
         self.order.copy_(torch.as_tensor(order))
+        if self.parts is not None:
+            self.items.copy_(torch.from_numpy(_split_items(order, self.parts)))
         self.graph.replay()
         return RaggedResult(u=self.u, v=self.v, niter=self.niter, status=self.status,
                             err_first=self.e_first, err_last=self.e_last,

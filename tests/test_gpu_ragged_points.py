@@ -260,3 +260,83 @@ def test_batch_autotuner_picks_a_compiled_config(tmp_path, monkeypatch):
         Pt = _plan(M, ut.cpu().numpy(), vt.cpu().numpy(), fam.regs[p])
         P8 = _plan(M, u8.cpu().numpy(), v8.cpu().numpy(), fam.regs[p])
         assert np.max(np.abs(Pt - P8)) / P8.max() < 1e-5, p
+
+
+def test_split_problems_fixed_iterations_match_oracle():
+    # Problems shared by 2 and 4 workgroups (uneven n, m, d < 16), against the
+    # float64 oracle at a fixed iteration count; the cost maxima are combined
+    # across the parts.
+    fam = problems.ragged_family(6, seed=77, n_range=(130, 700), d=5, kmin=1, kmax=3,
+                                 reg_range=(0.02, 0.2))
+    P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
+    ns = np.array([len(x) for x in fam.xs])
+    parts = np.where(ns >= 128, 4, 1)
+    parts[::3] = np.where(ns[::3] >= 64, 2, 1)
+    assert (parts > 1).sum() >= 4
+    it = 40
+    res = ragged.sinkhorn_log_ragged_points(P, numItermax=it, stopThr=0.0, split=parts)
+    torch.cuda.synchronize()
+    print("split fixed-iteration batch done", flush=True)
+    assert (res.niter.cpu().numpy() == it - 1).all()
+    assert (res.status.cpu().numpy() == 1).all()
+    cmax = res.cmax.cpu().numpy()
+    for p in range(P.batch):
+        Mraw = _cost(fam.xs[p], fam.ys[p], normalize=False)
+        assert abs(cmax[p] - Mraw.max()) <= 1e-12 * Mraw.max(), (p, cmax[p], Mraw.max())
+        M = Mraw / Mraw.max()
+        n, m = M.shape
+        a = np.full(n, 1.0 / n)
+        b = np.full(m, 1.0 / m)
+        Pr, lr = osk.sinkhorn_log(a, b, M, fam.regs[p], numItermax=it, stopThr=0.0, log=True,
+                                  warn=False)
+        u, v = res.split(P, p)
+        Pl = _plan(M, u.cpu().numpy(), v.cpu().numpy(), fam.regs[p])
+        mask = Pr >= 1e-30
+        rel = np.max(np.abs(Pl[mask] - Pr[mask]) / Pr[mask])
+        assert rel < 1e-5, (p, parts[p], rel)
+        np.testing.assert_allclose(res.err_first[p].item(), lr["err"][0], rtol=1e-4)
+
+
+def test_split_converged_warm_fallback_and_graph():
+    # "auto" split of a small uneven batch: converged plans agree with the
+    # one-workgroup-per-problem solve; a rejected warm start restarts cold in
+    # every part; a captured graph replays the split launch exactly.
+    fam = problems.ragged_family(16, seed=12, n_range=(64, 1024), d=16, kmin=1, kmax=4,
+                                 reg_range=(1e-2, 5e-2))
+    P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
+    assert (ragged.split_plan(P.ns, P.ms) > 1).sum() >= 4
+    r1 = ragged.sinkhorn_log_ragged_points(P, numItermax=2000, stopThr=1e-6, check_every=1)
+    rs = ragged.sinkhorn_log_ragged_points(P, numItermax=2000, stopThr=1e-6, check_every=1,
+                                           split="auto")
+    torch.cuda.synchronize()
+    print("split converged batch done", flush=True)
+    assert ((rs.status.cpu().numpy() & 3) == 0).all()
+    for p in range(P.batch):
+        M = _cost(fam.xs[p], fam.ys[p])
+        u1, v1 = r1.split(P, p)
+        us, vs = rs.split(P, p)
+        P1 = _plan(M, u1.cpu().numpy(), v1.cpu().numpy(), fam.regs[p])
+        Ps = _plan(M, us.cpu().numpy(), vs.cpu().numpy(), fam.regs[p])
+        assert np.max(np.abs(Ps - P1)) / P1.max() < 1e-4, p
+    # warm start far off: every problem falls back to the cold start
+    rng = np.random.default_rng(5)
+    u0 = torch.tensor(rng.normal(size=int(P.ns.sum())) * 50.0, device="cuda")
+    rw = ragged.sinkhorn_log_ragged_points(P, u0=u0, numItermax=2000, stopThr=1e-6,
+                                           check_every=1, fallback_thr=1e-3, split="auto")
+    torch.cuda.synchronize()
+    print("split fallback batch done", flush=True)
+    st = rw.status.cpu().numpy()
+    assert ((st & 3) == 0).all() and ((st & 4) > 0).all()
+    np.testing.assert_array_equal(rw.niter.cpu().numpy(), rs.niter.cpu().numpy())
+    ns = [len(x) for x in fam.xs]
+    ms = [len(y) for y in fam.ys]
+    G = ragged.RaggedGraph(ns, ms, 16, numItermax=2000, stopThr=1e-6, check_every=1,
+                           cost="points", split="auto")
+    for rep in range(2):
+        out = G.solve(P.X, P.Y, fam.regs)
+        torch.cuda.synchronize()
+        print("split graph replay", rep, "done", flush=True)
+        assert np.array_equal(out.niter.cpu().numpy(), rs.niter.cpu().numpy())
+        assert np.array_equal(out.u.cpu().numpy(), rs.u.cpu().numpy())
+        assert np.array_equal(out.v.cpu().numpy(), rs.v.cpu().numpy())
+        _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
