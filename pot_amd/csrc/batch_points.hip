@@ -238,6 +238,7 @@ struct PointsLds {
   double vsh[kPMax];              // column potentials (base 2)
   double yn[kPMax];               // s |y_c|^2
   double lbs[kPMax];              // log2 b_c (-inf where b_c = 0)
+  double bs[kPMax];               // b_c (read by the marginal check of every iteration)
   float af[kPMax];                // row weights
   float sh[kPMax];                // shifted row sums (prologue: column maxima)
   float colx[kPMax];              // column sums
@@ -660,6 +661,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     const int c = col_of(j);
     if (h == 0 && c < m) {
       const double bj = bp[c];
+      L.bs[c] = bj;
       L.lbs[c] = bj > 0.0 ? log2(bj) : -__builtin_inf();
       const double g = (double)L.sh[c];
       L.vsh[c] = (bj > 0.0 && __builtin_isfinite(g)) ? log2(bj) - (g + log2((double)L.colx[c]))
@@ -768,7 +770,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     for (int j = 0; j < NS; ++j) {
       const int c = col_of(j);
       if (h == 0 && c < m) {
-        const double dd = (double)L.colx[c] - bp[c];
+        const double dd = (double)L.colx[c] - L.bs[c];
         e2 += dd * dd;
       }
     }
