@@ -74,6 +74,22 @@ __device__ __forceinline__ float bf16_val(unsigned short h) {
   return __uint_as_float((unsigned int)h << 16);
 }
 
+// x = q[0] + q[1] + q[2] + q[3] to 2^-32 relative, each part a bf16, packed
+// two per word; a non-finite x gives (x, 0, 0, 0).
+__device__ __forceinline__ u32x2 split4_bf16(double x) {
+  const bool fin = __builtin_isfinite(x);
+  unsigned q[4];
+  double r = x;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const unsigned short b = bf16_rne((float)r);
+    q[i] = (i == 0 || fin) ? b : 0u;
+    r -= (double)bf16_val(b);
+  }
+  return u32x2{q[0] | (q[1] << 16), q[2] | (q[3] << 16)};
+}
+constexpr unsigned kOnes = 0x3F803F80u;  // two bf16 1.0
+
 // A point coordinate as double; points are fp32 or fp64 (x64) in HBM.
 __device__ __forceinline__ double coord(const void* p, int64_t idx, int x64) {
   return x64 ? static_cast<const double*>(p)[idx] : (double)static_cast<const float*>(p)[idx];
@@ -216,11 +232,17 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
   constexpr int NCHUNK = kStep * CPR;         // chunks per stage
   constexpr int NPT = (NCHUNK + 255) / 256;   // chunks per thread
   constexpr int NB = D * es / 32;             // 16-byte chunks per lane (half a point)
+  // bf16 operands: the potential term enters the accumulator through one
+  // more bf16 MFMA (K slots [P1..P4, 1, 1, 1, 1] x [1, 1, 1, 1, -Q1..-Q4],
+  // P the C point's term and Q the lane's shift, each split into four bf16
+  // parts), issued after the dot product.
+  constexpr bool KS = MODE == kModeLse && (P == POTAMD_ONLINE_BF16 || P == POTAMD_ONLINE_BF16X3);
 
   __shared__ __attribute__((aligned(16))) char sm_t[2][TILE];
   __shared__ __attribute__((aligned(16))) char sm_lo[LO ? 2 : 1][LO ? TILE : 16];
   __shared__ double sm_v[2][kStep];
   __shared__ float sm_sc[2][kStep];
+  __shared__ u32x2 sm_kp[KS ? 2 : 1][KS ? kStep : 1];
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
@@ -258,6 +280,14 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
       fresh = false;
     }
   }
+  // K slots of the lane's shift (B operand): [1, 1, 1, 1, -Q1..-Q4].
+  auto shift_slots = [&]() {
+    const u32x2 q = split4_bf16(-cl);
+    const u32x4 k = {kOnes, kOnes, q[0], q[1]};
+    return k;
+  };
+  u32x4 rkq = {0u, 0u, 0u, 0u};
+  if constexpr (KS) rkq = shift_slots();
   double ssum = 0.0;
   float best = -__builtin_inff();
   int best_c = 0;
@@ -267,6 +297,7 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
   u32x4 stl[LO ? NPT : 1];
   double st_v = 0.0;
   float st_sc = 1.f;
+  u32x2 st_kp = {0u, 0u};
   auto load = [&](int step) {
     const int64_t c0 = (int64_t)step * kStep;
     const char* cp = static_cast<const char*>(A.c_pk) + c0 * D * es;
@@ -293,6 +324,7 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
         st_v = -__builtin_inf();
       }
       if constexpr (FP8) st_sc = A.c_sc[c];
+      if constexpr (KS) st_kp = split4_bf16(st_v);
     }
   };
   auto store = [&](int buf) {
@@ -313,6 +345,7 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
     if (tid < kStep) {
       sm_v[buf][tid] = st_v;
       if constexpr (FP8) sm_sc[buf][tid] = st_sc;
+      if constexpr (KS) sm_kp[buf][tid] = st_kp;
     }
   };
 
@@ -322,7 +355,7 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
   auto tile = [&](int buf, int sb, double shift) -> f32x16 {
     f32x16 acc;
     const double* vrow = &sm_v[buf][sb * 32 + 4 * h];
-    if constexpr (MODE == kModeLse && !FP8) {
+    if constexpr (MODE == kModeLse && !FP8 && !KS) {
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -342,6 +375,17 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
         mma_chunk<P>(acc, al, rb[i]);
       }
       mma_chunk<P>(acc, a, rb[i]);
+    }
+    if constexpr (KS) {
+      // Output value i of lane (r, h) is C point 8(i/4) + 4h + i%4 of the
+      // tile, so the C points whose bit 2 is h take the K slots 8h..8h+7,
+      // which meet the shift slots of lane (r, h): each lane's outputs get
+      // its own shift.
+      const u32x2 kp = sm_kp[buf][sb * 32 + l32];
+      const bool mine = ((l32 >> 2) & 1) == h;
+      const u32x4 ka = {mine ? kp[0] : 0u, mine ? kp[1] : 0u, mine ? kOnes : 0u, mine ? kOnes : 0u};
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ka),
+                                                    __builtin_bit_cast(bf16x8, rkq), acc, 0, 0, 0);
     }
     if constexpr (MODE == kModeLse && FP8) {
       const float* srow = &sm_sc[buf][sb * 32 + 4 * h];
@@ -389,6 +433,7 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
               ssum *= exp2(-delta);
               cl += delta;
               fresh = false;
+              if constexpr (KS) rkq = shift_slots();
             }
             t = tile(buf, sb, cl);
             ts = 0.f;
