@@ -90,6 +90,19 @@ __device__ __forceinline__ u32x2 split4_bf16(double x) {
 }
 constexpr unsigned kOnes = 0x3F803F80u;  // two bf16 1.0
 
+// t <- exp2(t) elementwise; returns the sum of the 16 values as a pairwise
+// tree.
+__device__ __forceinline__ float exp2_tree_sum(f32x16& t) {
+#pragma unroll
+  for (int i = 0; i < 16; ++i) t[i] = exp2_fast(t[i]);
+  float s[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) s[i] = t[2 * i] + t[2 * i + 1];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) s[i] = s[2 * i] + s[2 * i + 1];
+  return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
 // A point coordinate as double; points are fp32 or fp64 (x64) in HBM.
 __device__ __forceinline__ double coord(const void* p, int64_t idx, int x64) {
   return x64 ? static_cast<const double*>(p)[idx] : (double)static_cast<const float*>(p)[idx];
@@ -412,15 +425,17 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
     for (int sb = 0; sb < 2; ++sb) {
       f32x16 t = tile(buf, sb, cl);
       if constexpr (MODE == kModeLse) {
-        float ts = 0.f;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) ts += exp2_fast(t[i]);
+        // exp2 in place (no extra registers), then a depth-4 tree sum, so
+        // the 16 exp2 of a lane are independent of each other.
+        float ts = exp2_tree_sum(t);
         // The tile sum itself flags a shift that is missing, too low (an
         // element above about +16: overflow risk) or too high while nothing
         // has been accumulated (everything below -40: underflow risk); only
-        // then is the tile maximum formed and the lane re-centred.
+        // then is the tile maximum formed (from the exponents, recomputed)
+        // and the lane re-centred.
         const bool suspect = fresh || !(ts <= 0x1p20f) || (ts < 0x1p-40f && ssum < 0x1p-40);
         if (__ballot(suspect)) {
+          t = tile(buf, sb, cl);
           float tmax = t[0];
 #pragma unroll
           for (int i = 1; i < 16; ++i) tmax = fmaxf(tmax, t[i]);
@@ -436,9 +451,7 @@ __global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
               if constexpr (KS) rkq = shift_slots();
             }
             t = tile(buf, sb, cl);
-            ts = 0.f;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) ts += exp2_fast(t[i]);
+            ts = exp2_tree_sum(t);
           }
         }
         ssum += (double)ts;
