@@ -233,8 +233,12 @@ __device__ __forceinline__ void mma_chunk(f32x16& acc, u32x4 a, u32x4 b) {
   }
 }
 
+// bf16x3 at d = 64 needs a few registers more than three waves per SIMD
+// allow; the pass is latency-bound there, so three waves are asked for.
 template <int P, int D, int MODE>
-__global__ __launch_bounds__(256) void online_pass_kernel(PassArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    (P == POTAMD_ONLINE_BF16X3 && D == 64 && MODE == kModeLse) ? 3 : 1)))
+void online_pass_kernel(PassArgs A) {
   if (A.ctrl && A.ctrl->done) return;
   constexpr int es = Op<P>::es;
   constexpr bool LO = Op<P>::lo;
