@@ -905,17 +905,42 @@ __device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int 
     double sx = 0.0, sy = 0.0;
     float lo = __builtin_inff(), hi = -__builtin_inff();
     if (k < d) {
-      for (int i = g; i < n; i += NG) {
-        const float x = Xp[(int64_t)i * d + k];
-        sx += x;
-        lo = fminf(lo, x);
-        hi = fmaxf(hi, x);
+      // kU points per thread are loaded before any is used (the index past
+      // the end is clamped to the group's first point, i0 < n), so their
+      // latencies overlap; they are summed in the same order as one at a
+      // time.
+      constexpr int kU = 16;
+      for (int i0 = g; i0 < n; i0 += NG * kU) {
+        float xv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int i = i0 + NG * u;
+          xv[u] = Xp[(int64_t)(i < n ? i : i0) * d + k];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          if (i0 + NG * u < n) {
+            sx += xv[u];
+            lo = fminf(lo, xv[u]);
+            hi = fmaxf(hi, xv[u]);
+          }
+        }
       }
-      for (int i = g; i < m; i += NG) {
-        const float y = Yp[(int64_t)i * d + k];
-        sy += y;
-        lo = fminf(lo, y);
-        hi = fmaxf(hi, y);
+      for (int i0 = g; i0 < m; i0 += NG * kU) {
+        float yv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int i = i0 + NG * u;
+          yv[u] = Yp[(int64_t)(i < m ? i : i0) * d + k];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          if (i0 + NG * u < m) {
+            sy += yv[u];
+            lo = fminf(lo, yv[u]);
+            hi = fmaxf(hi, yv[u]);
+          }
+        }
       }
     }
     scr[tid] = sx;
