@@ -545,28 +545,56 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     for (int r = tid; r < nblk * 32; r += PT)
       L.rks[r] = kslot_row(r < n ? L.xn[r] : -__builtin_inf());
     __syncthreads();
-    float best = __builtin_inff();
-    int code = -1;
+    // One running minimum per column tile (independent chains), merged at
+    // the end: the first occurrence in (blk, j, i) order wins, as with one
+    // running minimum, since code grows with that order.
+    float bestj[NS];
+    int codej[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      bestj[j] = __builtin_inff();
+      codej[j] = -1;
+    }
     for (int blk = blk0; blk < blk1; ++blk) {
       tiles(blk, row_kslot(blk));
 #pragma unroll
       for (int j = 0; j < NS; ++j) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-          const bool lt = e[j][i] < best;
-          best = lt ? e[j][i] : best;
-          code = lt ? (blk << 8) | (j << 4) | i : code;
+          const bool lt = e[j][i] < bestj[j];
+          bestj[j] = lt ? e[j][i] : bestj[j];
+          codej[j] = lt ? (blk << 8) | (j << 4) | i : codej[j];
         }
       }
+    }
+    float best = __builtin_inff();
+    int code = -1;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const bool take = codej[j] >= 0 && (bestj[j] < best || (bestj[j] == best && codej[j] < code));
+      best = take ? bestj[j] : best;
+      code = take ? codej[j] : code;
     }
     double mexact = 0.0;
     if (code >= 0) {
       const int blk = code >> 8, j = (code >> 4) & 15, i = code & 15;
       const int r = blk * 32 + l32, c = out_col(j, i);
       if (r < n && c < m) {
-        for (int k = 0; k < d; ++k) {
-          const double t = (double)Xp[(int64_t)r * d + k] - (double)Yp[(int64_t)c * d + k];
-          mexact = fma(t, t, mexact);
+        // All coordinates loaded first (d <= kPD, checked at launch; the
+        // index past d is clamped to 0), then summed in order.
+        float xs[kPD], ys[kPD];
+#pragma unroll
+        for (int k = 0; k < kPD; ++k) {
+          const int kk = k < d ? k : 0;
+          xs[k] = Xp[(int64_t)r * d + kk];
+          ys[k] = Yp[(int64_t)c * d + kk];
+        }
+#pragma unroll
+        for (int k = 0; k < kPD; ++k) {
+          if (k < d) {
+            const double t = (double)xs[k] - (double)ys[k];
+            mexact = fma(t, t, mexact);
+          }
         }
       }
     }
