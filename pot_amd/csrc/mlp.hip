@@ -139,13 +139,62 @@ __global__ void segment_mean_kernel(int seg_len, int h, const uint16_t* H, float
 // out[p][n] = sum_k in[p][k] W[k][n] (+ bias[n], or bias[p][n] when bias_per_row) --
 // small fp32 products with
 // the per-problem vectors (W fp32, row-major, k x n).
-__global__ void vec_mat_kernel(int P, int k, int n, const float* in, const float* W,
-                               const float* bias, int bias_per_row, float* out) {
-  const int p = blockIdx.x;
-  for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    float s = bias ? bias[(bias_per_row ? (int64_t)p * n : 0) + c] : 0.f;
-    for (int q = 0; q < k; ++q) s = fmaf(in[(int64_t)p * k + q], W[(int64_t)q * n + c], s);
-    out[(int64_t)p * n + c] = s;
+// kVecP problems per workgroup: their input rows pass through LDS kVecK
+// entries at a time, so each W element read serves kVecP problems.  Each
+// output is accumulated in q order, as one problem at a time would.
+constexpr int kVecP = 8;
+constexpr int kVecK = 256;
+__global__ __launch_bounds__(256) void vec_mat_kernel(int P, int k, int n, const float* in,
+                                                      const float* W, const float* bias,
+                                                      int bias_per_row, float* out) {
+  __shared__ __attribute__((aligned(16))) float xs[kVecP][kVecK];
+  const int p0 = blockIdx.x * kVecP;
+  for (int c0 = 0; c0 < n; c0 += blockDim.x) {
+    const int c = c0 + threadIdx.x;
+    float s[kVecP];
+#pragma unroll
+    for (int i = 0; i < kVecP; ++i) {
+      const int p = p0 + i < P ? p0 + i : P - 1;
+      s[i] = (bias && c < n) ? bias[(bias_per_row ? (int64_t)p * n : 0) + c] : 0.f;
+    }
+    for (int k0 = 0; k0 < k; k0 += kVecK) {
+      const int kc = k - k0 < kVecK ? k - k0 : kVecK;
+      __syncthreads();  // the previous chunk's readers are done
+      for (int e = threadIdx.x; e < kVecP * kVecK; e += blockDim.x) {
+        const int i = e / kVecK, q = e % kVecK;
+        const int p = p0 + i < P ? p0 + i : P - 1;
+        xs[i][q] = q < kc ? in[(int64_t)p * k + k0 + q] : 0.f;
+      }
+      __syncthreads();
+      if (c < n) {
+        int q = 0;
+        // four q at a time: their W loads issue together, the inputs are
+        // read from LDS as float4 (q and kVecK are multiples of 4)
+        for (; q + 4 <= kc; q += 4) {
+          float w[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) w[t] = W[(int64_t)(k0 + q + t) * n + c];
+#pragma unroll
+          for (int i = 0; i < kVecP; ++i) {
+            const float4 x = *reinterpret_cast<const float4*>(&xs[i][q]);
+            s[i] = fmaf(x.x, w[0], s[i]);
+            s[i] = fmaf(x.y, w[1], s[i]);
+            s[i] = fmaf(x.z, w[2], s[i]);
+            s[i] = fmaf(x.w, w[3], s[i]);
+          }
+        }
+        for (; q < kc; ++q) {
+          const float w = W[(int64_t)(k0 + q) * n + c];
+#pragma unroll
+          for (int i = 0; i < kVecP; ++i) s[i] = fmaf(xs[i][q], w, s[i]);
+        }
+      }
+    }
+    if (c < n) {
+#pragma unroll
+      for (int i = 0; i < kVecP; ++i)
+        if (p0 + i < P) out[(int64_t)(p0 + i) * n + c] = s[i];
+    }
   }
 }
 
@@ -179,9 +228,18 @@ __global__ void row_norm_kernel(int rows, int seg_len, int dx, const float* pts,
   const int p = r / seg_len;
   const float sc2 = 1.f / (cmax ? cmax[p] : 1.f);
   float s = 0.f;
-  for (int c = 0; c < dx; ++c) {
-    const float v = pts[(int64_t)r * dx + c];
-    s = fmaf(v, v, s);
+  const float* row = pts + (int64_t)r * dx;
+  if ((dx & 3) == 0 && (reinterpret_cast<uintptr_t>(pts) & 15) == 0) {
+    // 16-byte loads, the same order of accumulation
+    for (int c = 0; c < dx; c += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(row + c);
+      s = fmaf(v.x, v.x, s);
+      s = fmaf(v.y, v.y, s);
+      s = fmaf(v.z, v.z, s);
+      s = fmaf(v.w, v.w, s);
+    }
+  } else {
+    for (int c = 0; c < dx; ++c) s = fmaf(row[c], row[c], s);
   }
   qn[r] = s * sc2;
 }
@@ -324,7 +382,8 @@ hipError_t launch_mlp_op(const MlpOp& o, hipStream_t s) {
                          static_cast<const uint16_t*>(o.x), static_cast<float*>(o.out));
       break;
     case kOpVecMat:
-      hipLaunchKernelGGL(vec_mat_kernel, dim3(o.P), dim3(256), 0, s, o.P, o.k, o.n,
+      hipLaunchKernelGGL(vec_mat_kernel, dim3((o.P + kVecP - 1) / kVecP), dim3(256), 0, s,
+                         o.P, o.k, o.n,
                          static_cast<const float*>(o.x), static_cast<const float*>(o.w),
                          static_cast<const float*>(o.y), o.flag, static_cast<float*>(o.out));
       break;

@@ -253,6 +253,7 @@ class RaggedPoints:
     normalize: bool
     regs_host: np.ndarray = None  # (B,) float64 host copy of regs (launch order)
     cmax: torch.Tensor = None  # (B,) float64 cost maxima once found (points_cmax)
+    order: torch.Tensor = None  # (B,) int32 longest-first launch order, on the device
 
     @property
     def batch(self):
@@ -301,12 +302,16 @@ def points_from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None
     t = lambda x, dt: torch.tensor(x, dtype=dt, device=device)
     regs_h = np.asarray(regs, dtype=np.float64).reshape(-1)
     regs_t = torch.from_numpy(regs_h.copy()).to(device)
+    # The launch order is uploaded here, with the other host-built tables: an
+    # upload at solve time would wait for the work already queued on the
+    # stream (a warm start's network) before the solve could be launched.
+    order_t = torch.from_numpy(_points_order(ns, ms, regs_h)).to(device)
     return RaggedPoints(device=device, ns=ns, ms=ms, d=d, X=_pack_points(xs, device),
                         Y=_pack_points(ys, device), aoff=t(aoff, torch.int64),
                         boff=t(boff, torch.int64), ns_d=t(ns, torch.int32),
                         ms_d=t(ms, torch.int32), a=_weights(a, ns, device),
                         b=_weights(b, ms, device), regs=regs_t, normalize=bool(normalize),
-                        regs_host=regs_h)
+                        regs_host=regs_h, order=order_t)
 
 
 def _check_order(order, B):
@@ -434,7 +439,9 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
     given = cmax is not None and P.normalize
     cmax = cmax.to(device=dev, dtype=torch.float64).contiguous() if given \
         else torch.empty(B, dtype=torch.float64, device=dev)
-    if order is None:
+    if order is None and P.order is not None:
+        order = P.order
+    elif order is None:
         regs_h = P.regs_host if P.regs_host is not None else P.regs.cpu().numpy()
         order = torch.from_numpy(_points_order(P.ns, P.ms, regs_h)).to(dev)
     else:
