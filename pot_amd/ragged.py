@@ -93,6 +93,10 @@ def _layout(ns, ms, device):
 def _weights(ws, counts, device):
     if ws is None:
         counts = np.asarray(counts, dtype=np.int64)
+        if np.all(counts == counts[0]):
+            # one size: a single fill with the same value 1 / n
+            return torch.full((int(counts.sum()),), 1.0 / float(counts[0]), dtype=torch.float64,
+                              device=device)
         inv = torch.from_numpy(1.0 / counts.astype(np.float64)).to(device)
         return torch.repeat_interleave(inv, torch.from_numpy(counts).to(device),
                                        output_size=int(counts.sum()))
@@ -297,21 +301,32 @@ def points_from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None
         raise ValueError("every problem needs at least one point on each side")
     if ms.max() > MAX_M or ns.max() > MAX_N:
         raise ValueError(f"ragged solver supports n <= {MAX_N}, m <= {MAX_M}")
+    B = len(ns)
     aoff = np.concatenate([[0], np.cumsum(ns)[:-1]])
     boff = np.concatenate([[0], np.cumsum(ms)[:-1]])
-    t = lambda x, dt: torch.tensor(x, dtype=dt, device=device)
     regs_h = np.asarray(regs, dtype=np.float64).reshape(-1)
-    regs_t = torch.from_numpy(regs_h.copy()).to(device)
-    # The launch order is uploaded here, with the other host-built tables: an
-    # upload at solve time would wait for the work already queued on the
-    # stream (a warm start's network) before the solve could be launched.
-    order_t = torch.from_numpy(_points_order(ns, ms, regs_h)).to(device)
+    if regs_h.size == 1:
+        regs_h = np.full(B, regs_h[0])
+    if regs_h.size != B:
+        raise ValueError(f"regs has {regs_h.size} entries for {B} problems")
+    # The host-built tables (offsets, sizes, regs, and the launch order, which
+    # an upload at solve time would make wait for the work already queued on
+    # the stream, such as a warm start's network) go up in one copy.
+    host = np.empty(6 * B, dtype=np.int64)
+    host[0:B] = aoff
+    host[B:2 * B] = boff
+    host[2 * B:3 * B] = ns
+    host[3 * B:4 * B] = ms
+    host[4 * B:5 * B] = regs_h.view(np.int64)
+    host[5 * B:6 * B] = _points_order(ns, ms, regs_h)
+    tab = torch.from_numpy(host).to(device)
     return RaggedPoints(device=device, ns=ns, ms=ms, d=d, X=_pack_points(xs, device),
-                        Y=_pack_points(ys, device), aoff=t(aoff, torch.int64),
-                        boff=t(boff, torch.int64), ns_d=t(ns, torch.int32),
-                        ms_d=t(ms, torch.int32), a=_weights(a, ns, device),
-                        b=_weights(b, ms, device), regs=regs_t, normalize=bool(normalize),
-                        regs_host=regs_h, order=order_t)
+                        Y=_pack_points(ys, device), aoff=tab[0:B], boff=tab[B:2 * B],
+                        ns_d=tab[2 * B:3 * B].to(torch.int32),
+                        ms_d=tab[3 * B:4 * B].to(torch.int32), a=_weights(a, ns, device),
+                        b=_weights(b, ms, device), regs=tab[4 * B:5 * B].view(torch.float64),
+                        normalize=bool(normalize), regs_host=regs_h,
+                        order=tab[5 * B:6 * B].to(torch.int32))
 
 
 def _check_order(order, B):
