@@ -340,3 +340,31 @@ def test_split_converged_warm_fallback_and_graph():
         assert np.array_equal(out.u.cpu().numpy(), rs.u.cpu().numpy())
         assert np.array_equal(out.v.cpu().numpy(), rs.v.cpu().numpy())
         _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+
+
+def test_points_batch_tables():
+    """One upload carries offsets, sizes, regs and the launch order; a scalar
+    reg applies to every problem and a reg count that does not match the
+    batch is rejected (the kernel reads one reg per problem)."""
+    rng = np.random.default_rng(5)
+    xs = [rng.standard_normal((n, 3)).astype(np.float32) for n in (40, 70, 33)]
+    ys = [rng.standard_normal((m, 3)).astype(np.float32) for m in (50, 20, 64)]
+    P = ragged.points_from_clouds(xs, ys, [0.1, 0.05, 0.2])
+    np.testing.assert_array_equal(P.aoff.cpu().numpy(), [0, 40, 110])
+    np.testing.assert_array_equal(P.boff.cpu().numpy(), [0, 50, 70])
+    np.testing.assert_array_equal(P.ns_d.cpu().numpy(), [40, 70, 33])
+    np.testing.assert_array_equal(P.ms_d.cpu().numpy(), [50, 20, 64])
+    np.testing.assert_array_equal(P.regs.cpu().numpy(), [0.1, 0.05, 0.2])
+    # longest first by n m / reg: 40*50/0.1 = 2e4, 70*20/0.05 = 2.8e4, 33*64/0.2 = 1.056e4
+    np.testing.assert_array_equal(P.order.cpu().numpy(), [1, 0, 2])
+    assert P.ns_d.dtype == torch.int32 and P.order.dtype == torch.int32
+    np.testing.assert_allclose(P.a.cpu().numpy()[:40], 1 / 40)
+    np.testing.assert_allclose(P.a.cpu().numpy()[40:110], 1 / 70)
+    Ps = ragged.points_from_clouds(xs, ys, 0.1)
+    np.testing.assert_array_equal(Ps.regs.cpu().numpy(), [0.1, 0.1, 0.1])
+    with pytest.raises(ValueError):
+        ragged.points_from_clouds(xs, ys, [0.1, 0.2])
+    # equal sizes: uniform weights from one fill, the same values
+    Pe = ragged.points_from_clouds(xs[:1] * 2, ys[:1] * 2, 0.1)
+    np.testing.assert_array_equal(Pe.a.cpu().numpy(), np.full(80, 1.0 / 40))
+    np.testing.assert_array_equal(Pe.b.cpu().numpy(), np.full(100, 1.0 / 50))
