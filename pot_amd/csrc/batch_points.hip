@@ -210,23 +210,6 @@ __device__ __forceinline__ float halves_max(float x) {
   return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
 }
 
-// Reduction over the 32 lanes of each wave half (the lanes that share a
-// column of an MFMA tile): four row-local DPP steps, then row 0 into row 1
-// and row 2 into row 3 (row_bcast:15).  Lanes 16-31 and 48-63 end with the
-// totals of their halves.
-template <bool MAX>
-__device__ __forceinline__ float half_reduce(float x) {
-  auto op = [](float p, float q) { return MAX ? fmaxf(p, q) : p + q; };
-  x = op(x, dpp_f<0xB1>(x));
-  x = op(x, dpp_f<0x4E>(x));
-  x = op(x, dpp_f<0x141>(x));
-  x = op(x, dpp_f<0x140>(x));
-  const float id = MAX ? -__builtin_inff() : 0.f;
-  const float y = __int_as_float(
-      __builtin_amdgcn_update_dpp(__float_as_int(id), __float_as_int(x), 0x142, 0xA, 0xF, false));
-  return op(x, y);
-}
-
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 
 template <int PW>
@@ -517,20 +500,41 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   };
   // Per-column reduction of acc over the rows (sum or max) into dst.
   f32x16 acc[NS];
+  // Recursive halving: an exchange with lane ^ 1 pairs values i and i+1
+  // (each lane keeps one of the two, summed over the pair), one with lane ^ 2
+  // pairs the results again, so 4 values per lane remain, value 4r + (lane &
+  // 3) in slot r; rotations by 4 and 8 within the 16-lane row sum each slot
+  // over the row, and a swap of rows 0 and 1 (2 and 3) adds the other row.
+  // Lanes l32 < 16 then store value 4 (l32 >> 2) + (l32 & 3) of the tile.
   auto columns_out = [&](auto max_tag, float* dst) {
     constexpr bool MAX = decltype(max_tag)::value;
+    auto op = [](float x, float y) { return MAX ? fmaxf(x, y) : x + y; };
+    const bool b0 = lane & 1, b1 = (lane >> 1) & 1;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
+      float s1[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float x0 = acc[j][2 * q], x1 = acc[j][2 * q + 1];
+        s1[q] = op(b0 ? x1 : x0, dpp_f<0xB1>(b0 ? x0 : x1));
+      }
+      float s2[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float x0 = s1[2 * r], x1 = s1[2 * r + 1];
+        s2[r] = op(b1 ? x1 : x0, dpp_f<0x4E>(b1 ? x0 : x1));
+      }
       float w = 0.f;
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const float t = half_reduce<MAX>(acc[j][i]);
-        w = (l32 == 16 + i) ? t : w;
+      for (int r = 0; r < 4; ++r) {
+        float y = op(s2[r], dpp_f<0x124>(s2[r]));  // row_ror:4
+        y = op(y, dpp_f<0x128>(y));                // row_ror:8
+        const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(y), __float_as_uint(y),
+                                                         false, false);
+        const float tot = op(__uint_as_float(sw[0]), __uint_as_float(sw[1]));
+        w = (l32 >> 2) == r ? tot : w;
       }
-      if (l32 >= 16) {
-        const int i = l32 - 16;
-        dst[32 * (wave + PW * j) + 8 * (i / 4) + 4 * h + (i % 4)] = w;
-      }
+      if (l32 < 16) dst[32 * (wave + PW * j) + 8 * (l32 >> 2) + 4 * h + (l32 & 3)] = w;
     }
   };
 
