@@ -282,7 +282,8 @@ def points_from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None
     """Pack point-cloud pairs for ``sinkhorn_log_ragged_points``.
 
     xs[p]: (n_p, d), ys[p]: (m_p, d) arrays or tensors (or (B, n, d) tensors
-    for equal sizes), n_p, m_p <= 1024, d <= 16; regs: (B,).  With
+    for equal sizes), n_p, m_p <= 1024, d <= 16; regs: (B,), or one value
+    for every problem (any other length raises ValueError).  With
     ``normalize`` each problem's cost is |x - y|^2 / max |x - y|^2.
     """
     device = device or _device.require_gpu()
