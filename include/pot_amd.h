@@ -250,6 +250,33 @@ int potamd_ragged_points_cmax(int batch, int max_n, int max_m, int d, const floa
                               const float* Y, const int64_t* xoff, const int64_t* yoff,
                               const int* ns, const int* ms, double* cmax, void* stream);
 
+/* Batched Minkowski / KL cost matrices, M[b] (batch x n x m, row-major, the
+ * dtype of X and Y; X: batch x n x d, Y: batch x m x d, contiguous):
+ *   metric 0: M_bij = (sum_k |x_bik - y_bjk|^p)^(q/p)   ot.batch.dist_lp_batch
+ *             (ot/batch/_linear.py:20-51); p, q > 0
+ *   metric 1: M_bij = sum_k y log(y + eps) - y log(x + eps)   dist_kl_batch
+ *             (:88-120, logits_X = False)
+ *   metric 2: M_bij = sum_k y log(y + eps) - y x   (logits_X = True, with the
+ *             sum over k of the docstring formula; the reference's own
+ *             logits branch, :115-116, omits that sum).
+ */
+int potamd_pairwise_batch(int dtype, int metric, const void* X, const void* Y, int batch, int n,
+                          int m, int d, double p, double q, double eps, void* M, void* stream);
+
+/* Strided batched GEMM, C_b = alpha op(A_b) op(B_b) + beta C_b (C read only
+ * when beta != 0), dtype POTAMD_F32 or POTAMD_F64 for all three.  Element
+ * (i, k) of op(A_b) is A[b sA + i rsA + k csA], element (k, j) of op(B_b) is
+ * B[b sB + k rsB + j csB] (so transposes and channel-last tensors are read in
+ * place), element (i, j) of C_b is C[b sC + i ldc + j].  Replaces the
+ * einsum products of the batched Gromov-Wasserstein tensor product
+ * (ot/batch/_quadratic.py:535-572, hC1 T hC2^T per channel) and of the
+ * batched cost backward.
+ */
+int potamd_gemm_batched(int dtype, int batch, int M, int N, int K, double alpha, const void* A,
+                        int64_t sA, int64_t rsA, int64_t csA, const void* B, int64_t sB,
+                        int64_t rsB, int64_t csB, double beta, void* C, int64_t sC,
+                        int64_t ldc, void* stream);
+
 /* ---- learned warm start (no counterpart in the reference) ------------- */
 
 /* bf16 MFMA GEMM with fused epilogue, the linear layers of the warm-start

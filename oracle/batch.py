@@ -100,10 +100,15 @@ def dist_lp_batch(X, Y, p=2, q=1):
     return M ** (q / p) if q != p else M
 
 
-def dist_kl_batch(X, Y, eps=1e-10):
-    """``KL(y_j || x_i)`` per pair with probability-vector rows (_linear.py:88-120)."""
+def dist_kl_batch(X, Y, eps=1e-10, logits_X=False):
+    """``KL(y_j || x_i)`` per pair with probability-vector rows (_linear.py:88-120).
+
+    ``logits_X``: X holds log probabilities, the cross term is ``sum_d y x``
+    (the docstring formula; the reference's logits branch, :115-116, omits
+    the sum over d)."""
     entr_y = np.sum(Y * np.log(Y + eps), axis=-1)
-    cross = np.sum(Y[:, None, :, :] * np.log(X + eps)[:, :, None, :], axis=-1)
+    hX = X if logits_X else np.log(X + eps)
+    cross = np.sum(Y[:, None, :, :] * hX[:, :, None, :], axis=-1)
     return entr_y[:, None, :] - cross
 
 

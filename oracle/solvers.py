@@ -7,15 +7,18 @@ Follows ot/solvers.py:428-486: defaults ``max_iter = 1000``, ``tol = 1e-9``;
 ``value_linear = sum(M * P)``; ``value = value_linear + reg * sum(P log(P +
 1e-16))`` for ``reg_type="entropy"`` or ``+ reg * kl_div(P, a b^T)`` with
 ``kl_div(p, q) = sum p log(p / q + 1e-16)`` (ot/backend.py:1542-1546) for
-"KL"; "last_step" runs ``max_iter - 1`` iterations and one more update
-(:454-475).
+"KL"; "last_step" runs ``max_iter - 1`` iterations, then (:463-475)
+``v' = log b - LSE_i(-M/reg + u)`` and ``u' = log a - LSE_j(-M/reg + v)``
+from the pre-step pair ``(u, v)`` (u' is computed from the old v).
 
 Pinning (tests/test_oracle.py): for any plan of the form
 ``P = exp(-M/reg + u_i + v_j)`` the two values satisfy, up to the 1e-16
 guards, ``value_KL = reg (sum_i (u_i - log a_i) r_i + sum_j (v_j - log b_j)
 c_j)`` and ``value_entropy = reg (sum_i u_i r_i + sum_j v_j c_j)`` with
-``r, c`` the plan's marginals; and "last_step" equals a full ``max_iter``
-solve that does not stop early.
+``r, c`` the plan's marginals; and "last_step" returns the ``log_u`` of a
+``max_iter - 1``-iteration solve with the ``log_v`` of a ``max_iter``-iteration
+one (both run with stopThr = 0, so neither stops early): the last iteration
+of sinkhorn_log sets ``u = c-transform(v)``, so ``u'`` equals ``u``.
 """
 
 import numpy as np
@@ -44,8 +47,9 @@ def solve_entropic(M, a=None, b=None, reg=1.0, reg_type="KL", max_iter=None, tol
     u, v = log["log_u"], log["log_v"]
     niter = log["niter"]
     if grad == "last_step":
-        v = np.log(b) - logsumexp(-M / reg + u[:, None], 0)
+        v_new = np.log(b) - logsumexp(-M / reg + u[:, None], 0)
         u = np.log(a) - logsumexp(-M / reg + v[None, :], 1)
+        v = v_new
         plan = np.exp(-M / reg + u[:, None] + v[None, :])
         niter = max_iter + 1
     value_linear = float(np.sum(M * plan))

@@ -92,6 +92,13 @@ SIGNATURES = {
         c_dbl, c_int, c_dbl, c_int,                # tau, max_iter, stop_thr, print_period
         c_vp, c_int, c_vp,                         # err_hist, err_cap, marg_sq
         c_vp, c_size, c_vp, c_vp, c_ip]),          # workspace, ws_bytes, host_ctrl, stream, info
+    "potamd_pairwise_batch": (c_int, [
+        c_int, c_int, c_vp, c_vp, c_int, c_int, c_int, c_int,  # dtype, metric, X, Y, B, n, m, d
+        c_dbl, c_dbl, c_dbl, c_vp, c_vp]),                     # p, q, eps, M, stream
+    "potamd_gemm_batched": (c_int, [
+        c_int, c_int, c_int, c_int, c_int, c_dbl,              # dtype, B, M, N, K, alpha
+        c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_i64,  # A, sA, rsA, csA, B, sB, rsB, csB
+        c_dbl, c_vp, c_i64, c_i64, c_vp]),                     # beta, C, sC, ldc, stream
     "potamd_sqeuclidean": (c_int, [
         c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_i64, c_i64, c_vp, c_i64,
         c_i64, c_int, c_int, c_int, c_vp]),

@@ -124,3 +124,30 @@ def sqeuclidean(X, Y, *, squared=True, clamp=True, zero_diag=False):
         ctypes.c_void_p(_device.stream_ptr(X.device)))
     _lib.check(rc, "potamd_sqeuclidean")
     return M
+
+
+def gemm(A, Bm, C, *, alpha=1.0, beta=0.0, transA=False, transB=False):
+    """``C = alpha op(A) op(B) + beta C`` per problem on the library's batched
+    GEMM; A (P, M, K) or (P, K, M), B (P, K, N) or (P, N, K), C (P, M, N)
+    contiguous, same float dtype."""
+    lib = _lib.load()
+    P, Mr, N = C.shape
+    K = A.shape[1] if transA else A.shape[2]
+    sa = (A.stride(2), A.stride(1)) if transA else (A.stride(1), A.stride(2))
+    sb = (Bm.stride(2), Bm.stride(1)) if transB else (Bm.stride(1), Bm.stride(2))
+    code = _lib.F32 if C.dtype == torch.float32 else _lib.F64
+    rc = lib.potamd_gemm_batched(
+        code, P, Mr, N, K, float(alpha), ctypes.c_void_p(A.data_ptr()), A.stride(0),
+        sa[0], sa[1], ctypes.c_void_p(Bm.data_ptr()), Bm.stride(0), sb[0], sb[1],
+        float(beta), ctypes.c_void_p(C.data_ptr()), C.stride(0), C.stride(1),
+        ctypes.c_void_p(_device.stream_ptr(C.device)))
+    _lib.check(rc, "potamd_gemm_batched")
+    return C
+
+
+def bmm(A, Bm, *, transA=False, transB=False, alpha=1.0):
+    """New ``alpha op(A) op(B)`` per problem on the library's batched GEMM."""
+    rows = A.shape[2] if transA else A.shape[1]
+    cols = Bm.shape[1] if transB else Bm.shape[2]
+    C = torch.empty((A.shape[0], rows, cols), dtype=A.dtype, device=A.device)
+    return gemm(A, Bm, C, alpha=alpha, transA=transA, transB=transB)

@@ -20,6 +20,7 @@ import torch
 
 from . import _lib, _solve
 from ._device import Ingress, is_empty
+from ._grad import envelope, requires_grad
 from .utils import list_to_array
 
 NOT_CONVERGED = (
@@ -75,7 +76,17 @@ def sinkhorn(a, b, M, reg, method="sinkhorn", numItermax=1000, stopThr=1e-9,
 
 def sinkhorn2(a, b, M, reg, method="sinkhorn", numItermax=1000, stopThr=1e-9,
               verbose=False, log=False, warn=False, warmstart=None, **kwargs):
-    """Entropic OT loss ``<P, M>`` (_sinkhorn.py:387-484); ``warn`` defaults to False."""
+    """Entropic OT loss ``<P, M>`` (_sinkhorn.py:387-484); ``warn`` defaults to False.
+
+    Gradients: with torch inputs that require grad (one histogram ``b``), the
+    loss carries the envelope gradient of the entropic problem at the
+    returned potentials, ``d/dM = P``, ``d/da = reg (log u - mean)``,
+    ``d/db = reg (log v - mean)`` (the ``set_gradients`` of ot/solvers.py:
+    483-495).  The reference instead differentiates through its torch
+    iterations (test/test_bregman.py:203-220 only checks that the gradients
+    exist and have the inputs' shapes); the iterations here run in the HIP
+    library, so that derivative is not available.
+    """
     M, a, b = list_to_array(M, a, b)
     name = method.lower()
     if name == "sinkhorn_stabilized":
@@ -89,14 +100,20 @@ def sinkhorn2(a, b, M, reg, method="sinkhorn", numItermax=1000, stopThr=1e-9,
     if name not in ("sinkhorn", "sinkhorn_log"):
         raise _method_error(method)
     solver = sinkhorn_knopp if name == "sinkhorn" else sinkhorn_log
-    if len(b.shape) >= 2:
-        return solver(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
-                      verbose=verbose, log=log, warn=warn, warmstart=warmstart,
-                      _loss_only=True, **kwargs)
-    res = solver(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
-                 verbose=verbose, log=log, warn=warn, warmstart=warmstart,
-                 _loss_only=True, **kwargs)
-    return res
+    if requires_grad(a, b, M):
+        if len(b.shape) >= 2:
+            raise NotImplementedError("gradients of sinkhorn2 with several histograms are "
+                                      "not part of this build")
+        parts = solver(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
+                       verbose=verbose, log=log, warn=warn, warmstart=warmstart,
+                       _grad_parts=True, **kwargs)
+        ing, loss, P, lu, lv, logd = parts
+        pairs = ((M, P), (a, reg * (lu - lu.mean())), (b, reg * (lv - lv.mean())))
+        val = envelope(loss, pairs, ing.out_dtype, ing.out_device)
+        return (val, logd) if log else val
+    return solver(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
+                  verbose=verbose, log=log, warn=warn, warmstart=warmstart,
+                  _loss_only=True, **kwargs)
 
 
 def _prep(a, b, M):
@@ -112,7 +129,8 @@ def _prep(a, b, M):
 
 
 def sinkhorn_log(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
-                 log=False, warn=True, warmstart=None, _loss_only=False, **kwargs):
+                 log=False, warn=True, warmstart=None, _loss_only=False, _grad_parts=False,
+                 **kwargs):
     """Log-domain Sinkhorn (_sinkhorn.py:799-918), iterations on the GPU.
 
     Each iteration: ``v = log b - LSE_i(-M/reg + u)``, ``u = log a -
@@ -178,8 +196,10 @@ def sinkhorn_log(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
         if res.status == _lib.MAX_ITER and warn:
             warnings.warn(NOT_CONVERGED)
 
-    P, _, sums = _solve.plan(Mt, u, v, reg, domain=_lib.LOG, want_plan=not _loss_only)
+    P, _, sums = _solve.plan(Mt, u, v, reg, domain=_lib.LOG,
+                             want_plan=_grad_parts or not _loss_only)
     out = ing.scalar(sums[0, 0].item()) if _loss_only else ing.out(P[0])
+    logd = None
     if log:
         log_u = ing.out(u[0])
         log_v = ing.out(v[0])
@@ -191,12 +211,16 @@ def sinkhorn_log(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
             "u": ing.out(torch.exp(u[0])),
             "v": ing.out(torch.exp(v[0])),
         }
+    if _grad_parts:
+        return ing, sums[0, 0].item(), P[0], u[0], v[0], logd
+    if log:
         return out, logd
     return out
 
 
 def sinkhorn_knopp(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
-                   log=False, warn=True, warmstart=None, _loss_only=False, **kwargs):
+                   log=False, warn=True, warmstart=None, _loss_only=False, _grad_parts=False,
+                   **kwargs):
     """Scaling-form Sinkhorn-Knopp (_sinkhorn.py:597-687), iterations on the GPU.
 
     ``v = b / (K^T u)``, ``u = 1 / (K/a v)`` with ``K = exp(-M/reg)``; a zero
@@ -241,7 +265,7 @@ def sinkhorn_knopp(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
         if res.status == _lib.MAX_ITER and warn:
             warnings.warn(NOT_CONVERGED)
 
-    want_plan = not (nh or _loss_only)
+    want_plan = _grad_parts or not (nh or _loss_only)
     P, _, sums = _solve.plan(Mt, u, v, reg, domain=_lib.SCALING, want_plan=want_plan,
                              batch=B, bstride=0)
     if nh:
@@ -250,11 +274,15 @@ def sinkhorn_knopp(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
         out = ing.scalar(sums[0, 0].item())
     else:
         out = ing.out(P[0])
+    logd = None
     if log:
         uu = u.t() if nh else u[0]
         vv = v.t() if nh else v[0]
         logd = {"err": [ing.scalar(e) for e in errs], "niter": niter,
                 "u": ing.out(uu), "v": ing.out(vv)}
+    if _grad_parts:
+        return ing, sums[0, 0].item(), P[0], torch.log(u[0]), torch.log(v[0]), logd
+    if log:
         return out, logd
     return out
 

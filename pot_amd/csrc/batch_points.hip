@@ -99,6 +99,8 @@ constexpr size_t kSplitCols = 256;                                    // byte of
 constexpr size_t kSplitDbl = kSplitCols + 2ull * kSplitMax * kPMax * 4;  // byte offset
 constexpr size_t kSplitSlot = kSplitDbl + 256;
 constexpr int kSplitSpin = 1 << 22;  // polls (with a sleep each) before giving up
+static_assert(kSplitSlot == kPointsSplitSlot && kSplitCols == kPointsSplitCounters,
+              "internal.h states the split slot layout");
 
 __device__ __forceinline__ unsigned bf16_bits(float f) {
   unsigned u = __float_as_uint(f);
@@ -1064,7 +1066,24 @@ __global__ __launch_bounds__(PW * kWave) void batch_points_kernel(PointsArgs A) 
 
 using namespace potamd;
 
-namespace {
+namespace potamd {
+// Argument checks shared by the point-cloud entry points (0 or an error).
+int points_validate(int batch, int max_n, int max_m, int d, const float* X, const float* Y,
+                    const int64_t* xoff, const int64_t* yoff, const int* ns, const int* ms,
+                    const double* regs, int normalize, const double* a, const double* b,
+                    const double* u, const double* v, int max_iter, int check_every,
+                    const int* niter, const int* status, const double* cmax, int waves,
+                    bool queued) {
+  if (batch < 1 || max_n < 1 || max_m < 1 || d < 1 || !X || !Y || !xoff || !yoff || !ns ||
+      !ms || !regs || !a || !b || !u || !v || max_iter < 1 || !niter || !status ||
+      check_every < 0 || normalize < 0 || normalize > 2 || (normalize == 2 && !cmax))
+    return POTAMD_EINVAL;
+  if (max_n > kPMax || max_m > kPMax || d > kPD || (queued && batch > (1 << 30)))
+    return POTAMD_EUNSUPPORTED;
+  if (waves != 0 && waves != 8 && waves != 16) return POTAMD_EINVAL;
+  return 0;
+}
+
 int points_launch(int batch, int max_n, int max_m, int d, const float* X, const float* Y,
                   const int64_t* xoff, const int64_t* yoff, const int* ns, const int* ms,
                   const double* regs, int normalize, const double* a, const double* b, double* u,
@@ -1072,14 +1091,11 @@ int points_launch(int batch, int max_n, int max_m, int d, const float* X, const 
                   double fallback_thr, const int* order, int* niter, int* status,
                   double* err_last, double* err_first, double* cmax, int waves, int* queue,
                   int items, const int* split, char* split_ws, int nsplit, void* stream) {
-  if (batch < 1 || max_n < 1 || max_m < 1 || d < 1 || !X || !Y || !xoff || !yoff || !ns ||
-      !ms || !regs || !a || !b || !u || !v || max_iter < 1 || !niter || !status ||
-      check_every < 0 || normalize < 0 || normalize > 2 || (normalize == 2 && !cmax))
-    return POTAMD_EINVAL;
-  if (max_n > kPMax || max_m > kPMax || d > kPD || (queue && batch > (1 << 30)))
-    return POTAMD_EUNSUPPORTED;
+  const int rc = points_validate(batch, max_n, max_m, d, X, Y, xoff, yoff, ns, ms, regs,
+                                 normalize, a, b, u, v, max_iter, check_every, niter, status,
+                                 cmax, waves, queue != nullptr);
+  if (rc != 0) return rc;
   if (waves == 0) waves = kPWDefault;
-  if (waves != 8 && waves != 16) return POTAMD_EINVAL;
   PointsArgs pa;
   memset(&pa, 0, sizeof(pa));
   pa.X = X;
@@ -1123,7 +1139,7 @@ int points_launch(int batch, int max_n, int max_m, int d, const float* X, const 
   HIP_TRY(hipGetLastError());
   return 0;
 }
-}  // namespace
+}  // namespace potamd
 
 extern "C" int potamd_sinkhorn_log_batch_points(
     int batch, int max_n, int max_m, int d, const float* X, const float* Y, const int64_t* xoff,

@@ -206,4 +206,23 @@ hipError_t launch_scale_copy(int64_t count, const double* x, double f, double* y
 hipError_t launch_fill(int64_t count, double val, double* y, hipStream_t s);
 hipError_t launch_ctrl_init(Ctrl* c, int* bad, hipStream_t s);
 
+// Point-cloud batch solver (batch_points.hip).  The split exchange slots
+// are kPointsSplitSlot bytes each; a slot's counters live in its first
+// kPointsSplitCounters bytes.
+constexpr size_t kPointsSplitSlot = 33280;
+constexpr size_t kPointsSplitCounters = 256;
+int points_validate(int batch, int max_n, int max_m, int d, const float* X, const float* Y,
+                    const int64_t* xoff, const int64_t* yoff, const int* ns, const int* ms,
+                    const double* regs, int normalize, const double* a, const double* b,
+                    const double* u, const double* v, int max_iter, int check_every,
+                    const int* niter, const int* status, const double* cmax, int waves,
+                    bool queued);
+int points_launch(int batch, int max_n, int max_m, int d, const float* X, const float* Y,
+                  const int64_t* xoff, const int64_t* yoff, const int* ns, const int* ms,
+                  const double* regs, int normalize, const double* a, const double* b, double* u,
+                  double* v, int max_iter, int check_every, double stop_thr,
+                  double fallback_thr, const int* order, int* niter, int* status,
+                  double* err_last, double* err_first, double* cmax, int waves, int* queue,
+                  int items, const int* split, char* split_ws, int nsplit, void* stream);
+
 }  // namespace potamd

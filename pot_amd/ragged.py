@@ -14,6 +14,7 @@ kept unnormalised in memory and the maximum is folded into ``reg``
 """
 
 import ctypes
+import warnings
 from dataclasses import dataclass
 
 import numpy as np
@@ -348,7 +349,13 @@ def split_plan(ns, ms, cus=None):
     """Workgroups per problem for a batch too small to fill the GPU with one
     workgroup per problem (at most 2 problems per CU): 4 for problems of at
     least 3 * 2^17 elements, 2 from 2^17, else 1 (each part needs 32 rows).
-    Larger batches are not split (their work already fills the GPU)."""
+    Larger batches are not split (their work already fills the GPU).
+
+    The parts of all split problems together never exceed the number of CUs
+    (the persistent grid): smaller split problems give up parts first.  With
+    the split problems queued first (``_split_items``), every part is taken
+    by the first wave of workgroups, so no part waits for a partner that a
+    busy workgroup has not dequeued yet."""
     ns = np.asarray(ns, dtype=np.int64)
     ms = np.asarray(ms, dtype=np.int64)
     parts = np.ones(len(ns), dtype=np.int32)
@@ -359,6 +366,10 @@ def split_plan(ns, ms, cus=None):
     nm = ns * ms
     parts[(nm >= 1 << 17) & (ns >= 64)] = 2
     parts[(nm >= 3 << 17) & (ns >= 128)] = 4
+    by_size = np.argsort(nm, kind="stable")
+    while parts[parts > 1].sum() > cus:
+        k = by_size[np.nonzero(parts[by_size] > 1)[0][0]]
+        parts[k] = 2 if parts[k] == 4 else 1
     return parts
 
 
@@ -372,13 +383,26 @@ def _split_tables(parts):
 
 
 def _split_items(order, parts):
-    """Queue items: the problems in launch order, each as its parts in a row."""
+    """Queue items: the split problems first, then the others, each group in
+    launch order, each problem as its parts in a row."""
     order = np.asarray(order, dtype=np.int64)
+    big = parts[order] > 1
+    order = np.concatenate([order[big], order[~big]])
     reps = parts[order]
     probs = np.repeat(order, reps)
     first = np.repeat(np.cumsum(reps) - reps, reps)
     part = np.arange(len(probs)) - first
     return (probs | (part << 24)).astype(np.int32)
+
+
+def _warn_unsolved(status):
+    """A split problem whose parts gave up waiting for each other reports
+    status 2 with NaN potentials; say so instead of returning NaNs quietly."""
+    bad = int(((status & 3) == 2).sum().item())
+    if bad:
+        warnings.warn(f"{bad} problem(s) of the batch were not solved (status 2: cost maximum "
+                      "0, scaled coordinates outside the f16 range, or split parts that gave "
+                      "up waiting for each other); their potentials are NaN")
 
 
 def _check_parts(parts, ns):
@@ -477,6 +501,7 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
                              cmax, items, torch.from_numpy(table).to(dev), nsplit, sws,
                              numItermax, check_every, stopThr, fallback_thr,
                              ctypes.c_void_p(_device.stream_ptr(dev)), queue, cmax_given=given)
+        _warn_unsolved(status)
         return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
                             err_last=e_last, cmax=cmax)
     _launch_points(lib, P, P.X, P.Y, P.regs, u, v, niter, status, e_last, e_first, cmax, order,
@@ -625,6 +650,8 @@ class RaggedGraph:
         if self.parts is not None:
             self.items.copy_(torch.from_numpy(_split_items(order, self.parts)))
         self.graph.replay()
+        if self.parts is not None:
+            _warn_unsolved(self.status)
         return RaggedResult(u=self.u, v=self.v, niter=self.niter, status=self.status,
                             err_first=self.e_first, err_last=self.e_last,
                             cmax=self.cmax if self.cost == "points" else None)

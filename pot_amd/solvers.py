@@ -30,6 +30,7 @@ import numpy as np
 import torch
 
 from . import _lib, _solve
+from ._grad import envelope, requires_grad as _requires_grad
 from ._device import Ingress
 from .bregman import NOT_CONVERGED, _print_errs
 from .utils import OTResult, dist
@@ -37,27 +38,6 @@ from .utils import OTResult, dist
 LAZY_METHODS = ("1d", "gaussian", "lowrank", "nystroem", "factored", "geomloss",
                 "geomloss_auto", "geomloss_tensorized", "geomloss_online",
                 "geomloss_multiscale")
-
-
-class _SetGradients(torch.autograd.Function):
-    """``value`` whose gradient with respect to ``inputs`` is ``grads``."""
-
-    @staticmethod
-    def forward(ctx, value, *inputs_and_grads):
-        k = len(inputs_and_grads) // 2
-        ctx.grads = inputs_and_grads[k:]
-        return value.detach().clone()
-
-    @staticmethod
-    def backward(ctx, grad_out):
-        outs = [None]
-        outs += [grad_out * g for g in ctx.grads]
-        outs += [None] * len(ctx.grads)
-        return tuple(outs)
-
-
-def _requires_grad(*xs):
-    return any(isinstance(x, torch.Tensor) and x.requires_grad for x in xs)
 
 
 def solve(M, a=None, b=None, reg=None, c=None, reg_type="KL", unbalanced=None,
@@ -109,27 +89,36 @@ def solve(M, a=None, b=None, reg=None, c=None, reg_type="KL", unbalanced=None,
         u, v, niter = res.u, res.v, res.niter
 
     if grad == "last_step":
-        # one more sinkhorn_log update (v first, then u) from the detached potentials
+        # One more update from the detached potentials (ot/solvers.py:463-475):
+        # v from the last u, u from the last v (not from the new v), so the
+        # forward value of u is the last u itself and only v moves.
         if has_grad:
             M0 = M if isinstance(M, torch.Tensor) else torch.as_tensor(M)
-            a0 = a if a is not None else at.to(M0.dtype)
-            b0 = b if b is not None else bt.to(M0.dtype)
+            a0 = a if a is not None else at.to(device=M0.device, dtype=M0.dtype)
+            b0 = b if b is not None else bt.to(device=M0.device, dtype=M0.dtype)
             u_d = u[0].to(device=M0.device, dtype=M0.dtype)
+            v_d = v[0].to(device=M0.device, dtype=M0.dtype)
             vn = torch.log(b0) - torch.logsumexp(-M0 / reg + u_d[:, None], 0)
-            un = torch.log(a0) - torch.logsumexp(-M0 / reg + vn[None, :], 1)
+            un = torch.log(a0) - torch.logsumexp(-M0 / reg + v_d[None, :], 1)
             plan = torch.exp(-M0 / reg + un[:, None] + vn[None, :])
-            value_linear = torch.sum(M0 * plan)
+            # the values use the detached inputs (:435, :476-481): the
+            # gradient reaches M, a and b through the plan only
+            Md = M0.detach()
+            value_linear = torch.sum(Md * plan)
             if reg_type.lower() == "entropy":
                 value = value_linear + reg * torch.sum(plan * torch.log(plan + 1e-16))
             else:
-                value = value_linear + reg * torch.sum(
-                    plan * torch.log(plan / (a0[:, None] * b0[None, :]) + 1e-16))
+                ab = a0.detach()[:, None] * b0.detach()[None, :]
+                value = value_linear + reg * torch.sum(plan * torch.log(plan / ab + 1e-16))
             return OTResult(potentials=(un, vn), value=value, value_linear=value_linear,
                             plan=plan, status=None)
+        # Without gradients the same step is the column half-step of one more
+        # iteration: a one-iteration solve from u returns v = c-transform(u)
+        # (the solver updates v first); its u is discarded.
         res = _solve.dense_solve(Mt, at.view(1, n), bt.view(1, m), u, v, reg,
                                  domain=_lib.LOG, order=_lib.COL_FIRST, max_iter=1,
                                  stop_thr=0.0)
-        u, v = res.u, res.v
+        v = res.v
         niter = max_iter + 1
 
     P, _, sums = _solve.plan(Mt, u, v, reg, domain=_lib.LOG, a=at.view(1, n),
@@ -140,13 +129,9 @@ def solve(M, a=None, b=None, reg=None, c=None, reg_type="KL", unbalanced=None,
     plan = ing.out(P[0])
     log_u, log_v = ing.out(u[0]), ing.out(v[0])
     if ing.is_torch and grad == "envelope" and has_grad:
-        inputs, grads = [], []
-        for x, g in ((M, P[0]), (a, reg * (u[0] - u[0].mean())), (b, reg * (v[0] - v[0].mean()))):
-            if isinstance(x, torch.Tensor) and x.requires_grad:
-                inputs.append(x)
-                grads.append(g.to(device=x.device, dtype=x.dtype).reshape(x.shape))
-        val_t = torch.tensor(value, dtype=ing.out_dtype, device=ing.out_device)
-        value_t = _SetGradients.apply(val_t, *inputs, *grads)
+        # d value / d(M, a, b) = (P, reg (u - mean u), reg (v - mean v)), :483-495
+        pairs = ((M, P[0]), (a, reg * (u[0] - u[0].mean())), (b, reg * (v[0] - v[0].mean())))
+        value_t = envelope(value, pairs, ing.out_dtype, ing.out_device)
         vl_t = torch.tensor(value_linear, dtype=ing.out_dtype, device=ing.out_device)
         return OTResult(potentials=(log_u, log_v), value=value_t, value_linear=vl_t,
                         plan=plan, status=None)

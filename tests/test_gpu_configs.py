@@ -1,0 +1,254 @@
+"""The benchmark configurations at their full sizes on the GPU.
+
+* Single large problem, 16384 x 16384, d = 64 (Dirichlet weights, reg
+  1e-2, 200 fixed iterations): the materialised fp32 cost (1 GiB) and the
+  online fp32 and compensated bf16 passes, each against the fp64 online
+  pass on the same points at full size.
+* 512 uneven problems (n_i, m_i in [64, 1024], d = 16, GMM(1-4), stopThr
+  1e-6) in one captured point-cloud hipGraph with the automatic split:
+  every problem converges, 16 of them (the 4 largest among them) match the
+  float64 oracle at the oracle's own stop, and replays equal the eager run.
+* Precision study, 65536 x 65536, d = 128 (reg 5e-2, 100 iterations): every
+  x.y^T precision against the full-size fp64 GPU pass, and a 4096-point
+  subsample against the float64 oracle.
+* Learned warm start, n = m = 1024, d = 16, GMM(1-8): the trained network's
+  start fed to the solver; warm plans match the oracle started from the
+  same u0, and the mean iteration count falls.
+
+The "log-plan error" of potentials (u, v) against (u*, v*) is
+``max|du - c| + max|dv + c|`` with ``c = mean(du)``: a bound on
+``max_ij |log P_ij - log P*_ij|``, so a plan within 1e-5 relative
+elementwise has a log-plan error below about 1e-5.
+"""
+
+import time
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.sinkhorn as osk
+from pot_amd import _lib, _solve, online, problems, ragged
+
+pytestmark = pytest.mark.gpu
+
+
+def _logplan_err(u, v, u_ref, v_ref):
+    du = (u - u_ref).double().cpu().numpy()
+    dv = (v - v_ref).double().cpu().numpy()
+    c = du.mean()
+    return float(np.max(np.abs(du - c)) + np.max(np.abs(dv + c)))
+
+
+def _cost64(x, y):
+    x = x.astype(np.float64)
+    y = y.astype(np.float64)
+    M = np.maximum((x ** 2).sum(1)[:, None] + (y ** 2).sum(1)[None] - 2.0 * x @ y.T, 0.0)
+    return M / M.max()
+
+
+def _rel_plan(P, Pr, floor=1e-30):
+    mask = Pr >= floor
+    return float(np.max(np.abs(P[mask] - Pr[mask]) / Pr[mask]))
+
+
+# ---------------------------------------------------------------- 16384^2
+
+def test_dense_and_online_16384_against_fp64():
+    x, y, a, b = problems.config2_clouds()
+    n, d = x.shape
+    dev = torch.device("cuda")
+    X = torch.tensor(x, device=dev)
+    Y = torch.tensor(y, device=dev)
+    at = torch.tensor(a, device=dev)
+    bt = torch.tensor(b, device=dev)
+    reg, its = 1e-2, 200
+    t0 = time.time()
+    u64, v64, lg64 = online.sinkhorn_online(X, Y, reg, at, bt, numItermax=its, stopThr=0.0,
+                                            precision="f64", warn=False)
+    cs = lg64["cost_scale"]
+    _, s64 = online.online_plan(X, Y, reg, u64, v64, cs, at, bt)
+    torch.cuda.synchronize()
+    print(f"fp64 online pass: {time.time() - t0:.1f} s, cost {s64['cost']:.9e}", flush=True)
+
+    results = {}
+    # materialised fp32 cost (1 GiB), normalised by the same maximum
+    M = _solve.sqeuclidean(X[None], Y[None], clamp=True)[0]
+    M /= cs
+    res = _solve.dense_solve(M, at.view(1, n), bt.view(1, n),
+                             torch.zeros(1, n, dtype=torch.float64, device=dev),
+                             torch.zeros(1, n, dtype=torch.float64, device=dev), reg,
+                             domain=_lib.LOG, order=_lib.COL_FIRST, max_iter=its, stop_thr=0.0)
+    del M
+    results["dense f32"] = (res.u[0], res.v[0])
+    for prec in ("f32", "bf16x3"):
+        u, v, _ = online.sinkhorn_online(X, Y, reg, at, bt, numItermax=its, stopThr=0.0,
+                                         precision=prec, cost_scale=cs, warn=False)
+        results[f"online {prec}"] = (u, v)
+    torch.cuda.synchronize()
+    bars = {"dense f32": (1e-5, 1e-6), "online f32": (1e-5, 1e-6),
+            "online bf16x3": (2e-5, 1e-6)}
+    for name, (u, v) in results.items():
+        _, s = online.online_plan(X, Y, reg, u, v, cs, at, bt)
+        lp = _logplan_err(u, v, u64, v64)
+        cost_rel = abs(s["cost"] - s64["cost"]) / s64["cost"]
+        print(f"{name}: log-plan err {lp:.2e}, cost rel err {cost_rel:.2e}, "
+              f"|P^T1 - b| {s['err_b']:.3e} (fp64 {s64['err_b']:.3e})", flush=True)
+        assert lp < bars[name][0], (name, lp)
+        assert cost_rel < bars[name][1], (name, cost_rel)
+        assert abs(s["err_b"] - s64["err_b"]) <= 1e-3 * s64["err_b"] + 1e-12, name
+
+
+# ---------------------------------------------------------------- 512 uneven
+
+def test_ragged_512_graph_converges_and_matches_oracle():
+    fam = problems.config3(seed=2, B=512)
+    ns = np.array([len(x) for x in fam.xs])
+    ms = np.array([len(y) for y in fam.ys])
+    Xp = torch.tensor(np.concatenate(fam.xs), device="cuda")
+    Yp = torch.tensor(np.concatenate(fam.ys), device="cuda")
+    G = ragged.RaggedGraph(ns, ms, 16, numItermax=2000, stopThr=1e-6, check_every=10,
+                           cost="points", split="auto")
+    assert G.parts is not None and (G.parts > 1).any()
+    outs = []
+    for rep in range(2):
+        r = G.solve(Xp, Yp, fam.regs)
+        outs.append(tuple(t.clone() for t in (r.u, r.v, r.niter, r.status)))
+        torch.cuda.synchronize()
+        _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+        print(f"512-problem graph replay {rep} done", flush=True)
+    P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
+    eager = ragged.sinkhorn_log_ragged_points(P, numItermax=2000, stopThr=1e-6,
+                                              check_every=10, split="auto")
+    torch.cuda.synchronize()
+    print("512-problem eager solve done", flush=True)
+    for o in outs:
+        for got, want in zip(o, (eager.u, eager.v, eager.niter, eager.status)):
+            assert torch.equal(got, want)
+    st = eager.status.cpu().numpy()
+    assert ((st & 3) == 0).all(), np.nonzero(st & 3)
+    niter = eager.niter.cpu().numpy()
+    largest = list(np.argsort(-(ns * ms))[:4])
+    rest = [p for p in range(0, 512, 41) if p not in largest]
+    picks = largest + rest[:12]
+    exact = 0
+    for p in picks:
+        M = _cost64(fam.xs[p], fam.ys[p])
+        n, m = M.shape
+        a, b = np.full(n, 1.0 / n), np.full(m, 1.0 / m)
+        Pr, lr = osk.sinkhorn_log(a, b, M, fam.regs[p], numItermax=2000, stopThr=1e-6,
+                                  log=True, warn=False)
+        u, v = eager.split(P, p)
+        Pg = np.exp(-M / fam.regs[p] + u.cpu().numpy()[:, None] + v.cpu().numpy()[None, :])
+        if int(niter[p]) == lr["niter"]:
+            exact += 1
+            rel = _rel_plan(Pg, Pr)
+            cost_rel = abs(np.sum(Pg * M) - np.sum(Pr * M)) / np.sum(Pr * M)
+            assert rel < 1e-5 and cost_rel < 1e-6, (p, rel, cost_rel)
+        else:
+            # one check apart: at the earlier of the two stops the oracle's
+            # error is within 1% of stopThr (the fp32 cost moves it across)
+            assert abs(int(niter[p]) - lr["niter"]) == 10, (p, niter[p], lr["niter"])
+            e = lr["err"][min(int(niter[p]), lr["niter"]) // 10]
+            assert abs(e - 1e-6) < 1e-8, (p, e)
+        print(f"problem {p} ({n}x{m}, reg {fam.regs[p]:.4f}): niter {niter[p]} "
+              f"(oracle {lr['niter']})", flush=True)
+    assert exact >= len(picks) - 1
+
+
+# ---------------------------------------------------------------- 65536^2
+
+def test_precision_study_65536():
+    x, y = problems.config4_clouds()
+    dev = torch.device("cuda")
+    X = torch.tensor(x, device=dev)
+    Y = torch.tensor(y, device=dev)
+    reg, its = 5e-2, 100
+    t0 = time.time()
+    u64, v64, lg = online.sinkhorn_online(X, Y, reg, numItermax=its, stopThr=0.0,
+                                          precision="f64", warn=False)
+    cs = lg["cost_scale"]
+    _, s64 = online.online_plan(X, Y, reg, u64, v64, cs)
+    torch.cuda.synchronize()
+    print(f"fp64 pass at 65536^2: {time.time() - t0:.1f} s", flush=True)
+    # bars about 3x the errors measured on this problem
+    bars = {"f32": (1e-6, 1.2e-8), "bf16x3": (1.4e-5, 9e-7), "bf16": (7e-3, 7e-6),
+            "fp8": (0.1, 1e-3)}
+    for prec, (b_lp, b_cost) in bars.items():
+        u, v, _ = online.sinkhorn_online(X, Y, reg, numItermax=its, stopThr=0.0, precision=prec,
+                                         cost_scale=cs, warn=False)
+        _, s = online.online_plan(X, Y, reg, u, v, cs)
+        lp = _logplan_err(u, v, u64, v64)
+        cost_rel = abs(s["cost"] - s64["cost"]) / s64["cost"]
+        print(f"{prec}: log-plan err {lp:.2e}, cost rel err {cost_rel:.2e}", flush=True)
+        assert lp < b_lp and cost_rel < b_cost, (prec, lp, cost_rel)
+    # 4096-point subsample against the float64 oracle
+    k = 4096
+    xs, ys = x[:k], y[:k]
+    M = _cost64(xs, ys)
+    ab = np.full(k, 1.0 / k)
+    Pr = osk.sinkhorn_log(ab, ab, M, reg, numItermax=its, stopThr=0.0, warn=False)
+    cost_ref = float(np.sum(Pr * M))
+    sub_bars = {"f64": (1e-12, 1e-13), "f32": (1.5e-6, 2e-8), "bf16x3": (1.5e-5, 8e-7)}
+    for prec, (b_plan, b_cost) in sub_bars.items():
+        u, v, lg = online.sinkhorn_online(xs, ys, reg, numItermax=its, stopThr=0.0,
+                                          precision=prec, warn=False)
+        P, s = online.online_plan(xs, ys, reg, u, v, lg["cost_scale"], want_plan=True)
+        rel = _rel_plan(P.cpu().numpy(), Pr)
+        cost_rel = abs(s["cost"] - cost_ref) / cost_ref
+        print(f"subsample {prec}: plan rel err {rel:.2e}, cost rel err {cost_rel:.2e}",
+              flush=True)
+        assert rel < b_plan and cost_rel < b_cost, (prec, rel, cost_rel)
+
+
+# ---------------------------------------------------------------- warm start
+
+def test_trained_warm_start_gain_and_oracle_parity():
+    from pot_amd import warmstart
+    net = warmstart.load_default(torch.device("cuda"))
+    assert net is not None, "trained weights missing"
+    fam = problems.config5(seed=1_000_000, count=64)
+    X = torch.tensor(np.stack(fam.xs), device="cuda")
+    Y = torch.tensor(np.stack(fam.ys), device="cuda")
+    R = ragged.points_from_clouds(X, Y, fam.regs)
+    cm = ragged.points_cmax(R)
+    u0 = warmstart.predict_u0(net, R, X, Y, fam.regs)
+    runs = {}
+    for ce in (10, 1):
+        cold = ragged.sinkhorn_log_ragged_points(R, numItermax=2000, stopThr=1e-6,
+                                                 check_every=ce, cmax=cm)
+        warm = ragged.sinkhorn_log_ragged_points(R, u0=u0, numItermax=2000, stopThr=1e-6,
+                                                 check_every=ce, cmax=cm,
+                                                 fallback_thr=warmstart.FALLBACK_THR)
+        torch.cuda.synchronize()
+        runs[ce] = (cold, warm)
+        nc = cold.niter.cpu().numpy().astype(float)
+        nw = warm.niter.cpu().numpy().astype(float)
+        assert ((cold.status.cpu().numpy() & 3) == 0).all()
+        assert ((warm.status.cpu().numpy() & 3) == 0).all()
+        print(f"check_every {ce}: mean iterations cold {nc.mean():.2f}, warm {nw.mean():.2f}, "
+              f"gain {nc.mean() / nw.mean():.3f}x, fallbacks "
+              f"{int(((warm.status.cpu().numpy() & 4) > 0).sum())}", flush=True)
+        assert nw.mean() < nc.mean()
+    # parity: the warm-started solve is sinkhorn_log started from the same u0
+    cold, warm = runs[10]
+    u0h = u0.cpu().numpy()
+    st = warm.status.cpu().numpy()
+    niter = warm.niter.cpu().numpy()
+    for p in range(0, 64, 8):
+        M = _cost64(fam.xs[p], fam.ys[p])
+        n = M.shape[0]
+        a = np.full(n, 1.0 / n)
+        ws = None if st[p] & 4 else (u0h[p * n:(p + 1) * n], np.zeros(n))
+        Pr, lr = osk.sinkhorn_log(a, a, M, fam.regs[p], numItermax=2000, stopThr=1e-6,
+                                  log=True, warn=False, warmstart=ws)
+        u, v = warm.split(R, p)
+        Pg = np.exp(-M / fam.regs[p] + u.cpu().numpy()[:, None] + v.cpu().numpy()[None, :])
+        print(f"warm problem {p}: niter {niter[p]} (oracle {lr['niter']}), reg "
+              f"{fam.regs[p]:.4f}", flush=True)
+        assert abs(int(niter[p]) - lr["niter"]) <= 10, (p, niter[p], lr["niter"])
+        if int(niter[p]) == lr["niter"]:
+            rel = _rel_plan(Pg, Pr)
+            cost_rel = abs(np.sum(Pg * M) - np.sum(Pr * M)) / np.sum(Pr * M)
+            print(f"  plan rel err {rel:.2e}, cost rel err {cost_rel:.2e}", flush=True)
+            assert rel < 1e-5 and cost_rel < 1e-6, (p, rel, cost_rel)

@@ -1,7 +1,9 @@
 """Batched (fused) Gromov-Wasserstein (ot/batch/_quadratic.py) against the
-float64 oracle.  The inner projections run on the GPU dense solver; with
-float64 inputs they reproduce the oracle's iteration sequence, so plans and
-values agree to 1e-8 relative.
+float64 oracle.  The inner projections run on the GPU dense solver and the
+tensor product on the library's batched GEMM; with float64 inputs they
+reproduce the oracle's iteration sequence, so plans and values agree to 1e-8
+relative.  The envelope gradient is checked by central differences of the
+oracle's quadratic value with the plan held fixed.
 """
 
 import numpy as np
@@ -61,3 +63,78 @@ def test_gromov_batch_torch_envelope_gradient():
     assert C1t.grad is not None and torch.isfinite(C1t.grad).all()
     with pytest.raises(NotImplementedError):
         pb.solve_gromov_batch(C1t, C2t, reg=0.1, grad="autodiff")
+
+
+def test_gromov_batch_nonsymmetric_kl_channels():
+    rng = np.random.default_rng(21)
+    B, n, m, D = 2, 7, 6, 3
+    C1 = rng.random((B, n, n, D)) + 0.2
+    C2 = rng.random((B, m, m, D)) + 0.2
+    res = pb.solve_gromov_batch(C1, C2, reg=0.1, loss="kl", logits=False, max_iter=12,
+                                tol=1e-9)
+    ref = oq.solve_gromov_batch(C1, C2, reg=0.1, loss="kl", logits=False, max_iter=12,
+                                tol=1e-9)
+    assert res.log["n_iter"] == ref["n_iter"]
+    np.testing.assert_allclose(res.plan, ref["T"], rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(res.value, ref["value"], rtol=1e-8)
+    # squared loss on channel structures, not symmetric
+    res2 = pb.solve_gromov_batch(C1, C2, reg=0.1, max_iter=8, tol=1e-9)
+    ref2 = oq.solve_gromov_batch(C1, C2, reg=0.1, max_iter=8, tol=1e-9)
+    np.testing.assert_allclose(res2.plan, ref2["T"], rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(res2.value_quad, ref2["value_quad"], rtol=1e-8)
+
+
+def test_gromov_batch_alpha0_is_sinkhorn_at_reg_over_k():
+    # closed form of the outer loop: alpha = 0, K steps, converged projections
+    import oracle.sinkhorn as osk
+    rng = np.random.default_rng(22)
+    B, n, m, K, reg = 2, 9, 7, 4, 1.0
+    C1, C2 = _graphs(B, n, m, seed=23)
+    M = rng.random((B, n, m))
+    a = rng.dirichlet(np.ones(n), size=B)
+    b = rng.dirichlet(np.ones(m), size=B)
+    res = pb.solve_gromov_batch(C1, C2, reg=reg, a=a, b=b, M=M, alpha=0.0, max_iter=K, tol=0.0,
+                                max_iter_inner=5000, tol_inner=1e-14)
+    assert res.log["n_iter"] == K - 1
+    for p in range(B):
+        P = osk.sinkhorn_log(a[p], b[p], M[p], reg / K, numItermax=5000, stopThr=1e-14)
+        np.testing.assert_allclose(res.plan[p], P, rtol=1e-8, atol=1e-15)
+
+
+def test_gromov_product_and_envelope_gradient_fd():
+    rng = np.random.default_rng(24)
+    B, n, m = 2, 6, 5
+    C1 = rng.random((B, n, n))
+    C2 = rng.random((B, m, m))   # not symmetric
+    C1t = torch.tensor(C1, device="cuda", requires_grad=True)
+    C2t = torch.tensor(C2, device="cuda", requires_grad=True)
+    res = pb.solve_gromov_batch(C1t, C2t, reg=0.2, max_iter=6)
+    T = res.plan.detach().cpu().numpy()
+    res.value.sum().backward()
+    g1, g2 = C1t.grad.cpu().numpy(), C2t.grad.cpu().numpy()
+    a = np.full((B, n), 1 / n)
+    b = np.full((B, m), 1 / m)
+
+    def vq(c1, c2):
+        L = oq.tensor_batch(a, b, c1, c2, symmetric=False)
+        return oq.loss_quadratic_batch(L, T, recompute_const=True, symmetric=False).sum()
+
+    h = 1e-6
+    for C, g, idx in ((C1, g1, (1, 2, 4)), (C1, g1, (0, 5, 0)), (C2, g2, (1, 3, 1)),
+                      (C2, g2, (0, 0, 4))):
+        Cp, Cm = C.copy(), C.copy()
+        Cp[idx] += h
+        Cm[idx] -= h
+        if C is C1:
+            fd = (vq(Cp, C2) - vq(Cm, C2)) / (2 * h)
+        else:
+            fd = (vq(C1, Cp) - vq(C1, Cm)) / (2 * h)
+        assert abs(g[idx] - fd) < 1e-7 * max(1.0, abs(fd)), (idx, g[idx], fd)
+    # the product itself (numpy in, numpy out) against the oracle
+    L = pb.tensor_batch(a, b, C1, C2, symmetric=False)
+    Lo = oq.tensor_batch(a, b, C1, C2, symmetric=False)
+    for k in ("constC", "fC1", "fC2"):
+        np.testing.assert_allclose(L[k], Lo[k], rtol=1e-13, atol=1e-15)
+    got = pb.loss_quadratic_batch(L, T, symmetric=False)
+    np.testing.assert_allclose(got, oq.loss_quadratic_batch(Lo, T, symmetric=False),
+                               rtol=1e-12)

@@ -3,15 +3,17 @@
 The oracle is ``oracle.sinkhorn.sinkhorn_log`` on ``M = dist(x, y) / M.max()``
 computed in float64 from the same float32-valued points.
 
-Tolerances (stated per precision of the x . y^T term):
+Tolerances (per precision of the x . y^T term) are about 3x the errors
+measured on this problem (384 x 320, d = 40, reg 5e-2, 60 iterations):
 * f64    : iteration count and error history equal the oracle's; plan
-           entries within 1e-9 relative, cost within 1e-10 relative.
-* f32    : the north-star bar, plan entries within 1e-5 relative and cost
-           within 1e-6 relative (fixed iteration count).
-* bf16x3 : plan within 1e-3 relative (compensated bf16, ~16 bits per operand).
-* bf16   : plan within 5e-2 relative (8-bit operands; a precision-study case).
-* fp8    : plan within 1.0 relative, cost within 5e-2 (3-bit operands; a
-           precision-study case, reported rather than relied on).
+           measured 1.1e-14 (bar 1e-12), cost 1.5e-16 (bar 1e-13).
+* f32    : plan 8.6e-7 (bar 3e-6), cost 2.6e-9 (bar 1e-8): inside the
+           1e-5 / 1e-6 bar of an fp32 result.
+* bf16x3 : plan 1.1e-5 (bar 3.5e-5), cost 4.7e-7 (bar 1.5e-6).  Compensated
+           bf16 carries about 16 bits per operand: at this size it sits just
+           above the 1e-5 plan bar (4.5e-6 at 65536^2, d = 128).
+* bf16   : plan 6.9e-3 (bar 2.1e-2), cost 1.7e-4 (bar 5e-4).
+* fp8    : plan 1.3e-1 (bar 0.4), cost 2.3e-3 (bar 7e-3).
 The measured errors are printed for the precision study.
 """
 
@@ -25,8 +27,8 @@ from pot_amd import _lib, _solve
 
 pytestmark = pytest.mark.gpu
 
-PLAN_TOL = {"f64": 1e-9, "f32": 1e-5, "bf16x3": 1e-3, "bf16": 5e-2, "fp8": 1.0}
-COST_TOL = {"f64": 1e-10, "f32": 1e-6, "bf16x3": 1e-4, "bf16": 1e-2, "fp8": 5e-2}
+PLAN_TOL = {"f64": 1e-12, "f32": 3e-6, "bf16x3": 3.5e-5, "bf16": 2.1e-2, "fp8": 0.4}
+COST_TOL = {"f64": 1e-13, "f32": 1e-8, "bf16x3": 1.5e-6, "bf16": 5e-4, "fp8": 7e-3}
 
 
 def _clouds(n, m, d, seed):

@@ -2,10 +2,11 @@
 
 Each problem must behave like a lone ``sinkhorn_log`` call on
 ``ot.dist(x, y) / M.max()`` evaluated in float64 from the same float32
-points: the plan within 1e-5 relative (entries >= 1e-30) at a fixed
-iteration count, the stop at the same check (one check of slack), the plan
-within 1e-4 of its largest entry at convergence, and the cost maximum that
-the kernel finds equal to the float64 maximum.
+points: the plan within 1e-5 relative (entries >= 1e-30) and the cost within
+1e-6 at a fixed iteration count and at convergence (the stop at the
+oracle's check; one check apart only when the oracle's error there is within
+1% of stopThr), and the cost maximum that the kernel finds equal to the
+float64 maximum.
 """
 
 import numpy as np
@@ -80,13 +81,21 @@ def test_points_converged_matches_oracle_small_reg():
         b = np.full(m, 1.0 / m)
         Pr, lr = osk.sinkhorn_log(a, b, M, reg, numItermax=3000, stopThr=1e-6, log=True,
                                   warn=False)
-        assert abs(int(niter[p]) - lr["niter"]) <= 10, (p, niter[p], lr["niter"])
         u, v = res.split(P, p)
         Pl = _plan(M, u.cpu().numpy(), v.cpu().numpy(), reg)
-        assert np.max(np.abs(Pl - Pr)) / Pr.max() < 1e-4, p
-        assert np.linalg.norm(Pl.sum(0) - b) < 2e-6
-        cost_rel = abs(np.sum(Pl * M) - np.sum(Pr * M)) / np.sum(Pr * M)
-        assert cost_rel < 1e-5, (p, cost_rel)
+        assert np.linalg.norm(Pl.sum(0) - b) < 1.1e-6
+        if int(niter[p]) == lr["niter"]:
+            # the same stop: the fp32 bar on the plan and the cost
+            mask = Pr >= 1e-30
+            rel = np.max(np.abs(Pl[mask] - Pr[mask]) / Pr[mask])
+            cost_rel = abs(np.sum(Pl * M) - np.sum(Pr * M)) / np.sum(Pr * M)
+            assert rel < 1e-5 and cost_rel < 1e-6, (p, rel, cost_rel)
+        else:
+            # one check apart only when the oracle's error at the earlier
+            # stop is within 1% of stopThr
+            assert abs(int(niter[p]) - lr["niter"]) == 10, (p, niter[p], lr["niter"])
+            e = lr["err"][min(int(niter[p]), lr["niter"]) // 10]
+            assert abs(e - 1e-6) < 1e-8, (p, e)
 
 
 def test_points_full_size_problem_matches_oracle():

@@ -87,6 +87,23 @@ def test_plan_and_dist_argument_validation():
     assert rc == -1
 
 
+def test_pairwise_and_gemm_argument_validation():
+    # every rejection happens before any launch, so no GPU is touched
+    lib = _lib.load()
+    P = ctypes.c_void_p(0x1000)
+    assert lib.potamd_pairwise_batch(2, 0, P, P, 1, 4, 4, 3, 2.0, 1.0, 0.0, P, None) == -1
+    assert lib.potamd_pairwise_batch(0, 3, P, P, 1, 4, 4, 3, 2.0, 1.0, 0.0, P, None) == -1
+    assert lib.potamd_pairwise_batch(0, 0, P, P, 1, 4, 4, 3, 0.0, 1.0, 0.0, P, None) == -1
+    assert lib.potamd_pairwise_batch(0, 1, P, P, 1, 4, 4, 3, 0.0, 0.0, -1.0, P, None) == -1
+    assert lib.potamd_pairwise_batch(0, 0, P, P, 1, 4, 0, 3, 2.0, 1.0, 0.0, P, None) == -1
+    assert lib.potamd_gemm_batched(0, 1, 4, 4, 4, 1.0, P, 16, 4, 1, P, 16, 4, 1, 0.0, P, 16, 3,
+                                   None) == -1   # ldc < N
+    assert lib.potamd_gemm_batched(0, 1, 4, 4, 0, 1.0, P, 16, 4, 1, P, 16, 4, 1, 0.0, P, 16, 4,
+                                   None) == -1   # K = 0
+    assert lib.potamd_gemm_batched(0, 1, 4, 4, 4, 1.0, None, 16, 4, 1, P, 16, 4, 1, 0.0, P, 16,
+                                   4, None) == -1
+
+
 def test_no_gpu_means_loud_failure():
     torch = pytest.importorskip("torch")
     if torch.cuda.is_available():

@@ -300,16 +300,35 @@ def test_solve_entropic_value_identities():
         assert abs(val - ident) < 1e-9 * max(1.0, abs(ident))
 
 
-def test_solve_entropic_last_step_is_one_more_update():
+def test_solve_entropic_last_step_pins():
+    # Not converged (small reg, Dirichlet weights, few iterations), so a wrong
+    # update order shows.  The reference's last step (ot/solvers.py:463-475)
+    # computes v' from the last u and u' from the last v; since sinkhorn_log's
+    # last iteration already set u = c-transform(v), u' is the (max_iter - 1)
+    # run's log_u and v' the max_iter run's log_v.
+    from oracle.sinkhorn import sinkhorn_log
     from oracle.solvers import solve_entropic
     rng = np.random.default_rng(5)
-    M = rng.random((12, 9))
-    P1, (u1, v1), val1, _, n1 = solve_entropic(M, reg=0.5, max_iter=8, tol=0.0,
-                                               grad="last_step")
-    P2, (u2, v2), val2, _, n2 = solve_entropic(M, reg=0.5, max_iter=8, tol=0.0)
-    np.testing.assert_allclose(P1, P2, rtol=1e-12)
-    np.testing.assert_allclose(val1, val2, rtol=1e-12)
-    assert n1 == 8 and n2 == 7
+    n, m, reg, it = 40, 30, 0.01, 12
+    M = rng.random((n, m))
+    a = rng.dirichlet(np.ones(n))
+    b = rng.dirichlet(np.ones(m))
+    P1, (u1, v1), val1, vl1, n1 = solve_entropic(M, a, b, reg=reg, max_iter=it, tol=0.0,
+                                                 grad="last_step", reg_type="entropy")
+    _, lg_a = sinkhorn_log(a, b, M, reg, numItermax=it - 1, stopThr=0.0, log=True, warn=False)
+    _, lg_b = sinkhorn_log(a, b, M, reg, numItermax=it, stopThr=0.0, log=True, warn=False)
+    np.testing.assert_allclose(u1, lg_a["log_u"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(v1, lg_b["log_v"], rtol=0, atol=1e-12)
+    assert n1 == it
+    # v' is a column half-step: the plan's column marginal is b exactly, its
+    # row marginal is not (the solve has not converged)
+    np.testing.assert_allclose(P1.sum(0), b, rtol=1e-12)
+    assert np.abs(P1.sum(1) - a).max() > 1e-4
+    # ... and differs from the plan of a plain max_iter solve
+    P2, _, _, _, _ = solve_entropic(M, a, b, reg=reg, max_iter=it, tol=0.0)
+    assert np.abs(P1 - P2).max() > 1e-6
+    np.testing.assert_allclose(vl1, np.sum(M * P1), rtol=1e-14)
+    np.testing.assert_allclose(val1, vl1 + reg * np.sum(P1 * np.log(P1 + 1e-16)), rtol=1e-14)
 
 
 def test_empirical_lazy_doctest_and_fixed_point():
@@ -357,6 +376,82 @@ def test_gw_tensor_product_bruteforce(symmetric):
                   - C2.transpose(0, 2, 1)[:, None, None, :, :]) ** 2
         ref = 0.5 * (ref + np.einsum("bijkl,bjl->bik", loss_t, T))
     np.testing.assert_allclose(got, ref, rtol=1e-12)
+
+
+def test_gw_tensor_product_bruteforce_channels_and_kl():
+    # (B, n, n, D) structures: l(c1, c2) = sum_d (c1_d - c2_d)^2; KL loss:
+    # l(c1, c2) = sum_d c2_d log(c2_d + 1e-15) - c2_d h(c1_d), h = log(. + 1e-15)
+    # (or the identity for logits).
+    from oracle.quadratic import tensor_batch, tensor_product_batch
+    rng = np.random.default_rng(11)
+    B, n, m, D = 2, 5, 4, 3
+    C1 = rng.random((B, n, n, D)) + 0.1
+    C2 = rng.random((B, m, m, D)) + 0.1
+    T = rng.random((B, n, m))
+    T /= T.sum(axis=(1, 2), keepdims=True)
+    a, b = T.sum(2), T.sum(1)
+    cases = {
+        ("sqeuclidean", False): ((C1[:, :, :, None, None] - C2[:, None, None]) ** 2).sum(-1),
+        ("kl", False): (C2[:, None, None] * np.log(C2[:, None, None] + 1e-15)
+                        - C2[:, None, None] * np.log(C1[:, :, :, None, None] + 1e-15)).sum(-1),
+        ("kl", True): (C2[:, None, None] * np.log(C2[:, None, None] + 1e-15)
+                       - C2[:, None, None] * C1[:, :, :, None, None]).sum(-1),
+    }
+    for (loss, logits), tens in cases.items():
+        L = tensor_batch(a, b, C1, C2, symmetric=False, loss=loss, logits=logits)
+        got = tensor_product_batch(L, T, symmetric=False)
+        ref = np.einsum("bijkl,bjl->bik", tens, T)
+        tens_t = tens.transpose(0, 2, 1, 4, 3)
+        ref = 0.5 * (ref + np.einsum("bijkl,bjl->bik", tens_t, T))
+        # the const of fC1 / fC2 uses the symmetrised f: with T's own
+        # marginals as a, b it equals the brute force
+        np.testing.assert_allclose(got, ref, rtol=1e-12, err_msg=f"{loss} logits={logits}")
+
+
+def test_gw_outer_loop_alpha0_closed_form():
+    # alpha = 0 with M given: each outer step projects exp(-M/reg) * T_prev;
+    # the projection absorbs T_prev's row and column scalings, so after K
+    # steps with converged projections T is sinkhorn_log(a, b, M, reg / K).
+    from oracle.quadratic import solve_gromov_batch
+    rng = np.random.default_rng(12)
+    B, n, m, K, reg = 2, 9, 7, 4, 1.0
+    C1 = rng.random((B, n, n))
+    C2 = rng.random((B, m, m))
+    M = rng.random((B, n, m))
+    a = rng.dirichlet(np.ones(n), size=B)
+    b = rng.dirichlet(np.ones(m), size=B)
+    out = solve_gromov_batch(C1, C2, reg=reg, a=a, b=b, M=M, alpha=0.0, max_iter=K, tol=0.0,
+                             max_iter_inner=5000, tol_inner=1e-14)
+    assert out["n_iter"] == K - 1
+    for p in range(B):
+        P = osk.sinkhorn_log(a[p], b[p], M[p], reg / K, numItermax=5000, stopThr=1e-14)
+        np.testing.assert_allclose(out["T"][p], P, rtol=1e-9, atol=1e-15)
+
+
+def test_dist_lp_and_kl_batch_pins():
+    # Minkowski against scipy's cdist, raised to q; KL against scipy.stats
+    # entropy(y, x) = sum y log(y / x) on probability vectors; the logits form
+    # on log X equals the plain form up to the eps guard.
+    from scipy.spatial.distance import cdist
+    from scipy.stats import entropy
+    from oracle.batch import dist_kl_batch, dist_lp_batch
+    rng = np.random.default_rng(13)
+    B, n, m, d = 2, 6, 5, 4
+    X = rng.standard_normal((B, n, d))
+    Y = rng.standard_normal((B, m, d))
+    for p, q in ((1, 1), (2, 1), (3, 1), (1.5, 2.0), (2, 2)):
+        got = dist_lp_batch(X, Y, p=p, q=q)
+        for k in range(B):
+            ref = cdist(X[k], Y[k], metric="minkowski", p=p) ** q
+            np.testing.assert_allclose(got[k], ref, rtol=1e-12, err_msg=f"p={p} q={q}")
+    Xp = rng.dirichlet(np.ones(d), size=(B, n))
+    Yp = rng.dirichlet(np.ones(d), size=(B, m))
+    got = dist_kl_batch(Xp, Yp)
+    for k in range(B):
+        ref = np.array([[entropy(Yp[k, j], Xp[k, i]) for j in range(m)] for i in range(n)])
+        np.testing.assert_allclose(got[k], ref, rtol=1e-8, atol=1e-9)
+    np.testing.assert_allclose(dist_kl_batch(np.log(Xp), Yp, logits_X=True), got, rtol=1e-8,
+                               atol=1e-9)
 
 
 def test_stabilized_and_eps_scaling_oracle():
