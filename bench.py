@@ -33,6 +33,13 @@ import numpy as np
 
 METRIC = "problems/sec to marginal err≤1e-6 (learned warm-start vs plain GPU Sinkhorn)"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per iteration of the 16384^2 fp32 dense solve, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE per dispatch (KiB) of the iteration's three kernels:
+# fetch 540584.37 (row sweep) + 8429.00 (merge) + 2.66 (decide), write
+# 16911.19 + 136.00 + 0.04, each scaled by the counters' reading on a 1 GiB
+# device copy (FETCH_SIZE 524315 KiB: x 2.000; WRITE_SIZE 1048576 KiB: x 1.000).
+DENSE_C2_TRAFFIC = (540584.37 + 8429.00 + 2.66) * 1024 * 2.000 \
+    + (16911.19 + 136.00 + 0.04) * 1024 * 1.000
 MFMA_PEAK_TFLOPS = {"f32": 157.3, "f16": 2500.0, "bf16": 2500.0, "bf16x3": 2500.0, "fp8": 5000.0, "f64": 78.6}
 
 
@@ -79,6 +86,123 @@ def cpu_baseline_problem(x, y, reg, stop, max_iter, budget_s=30.0):
                          numItermax=max_iter, stopThr=stop, log=True, warn=False)
     dt = time.perf_counter() - t0
     return dt, lg["niter"]
+
+
+def _cpu_worker_init():
+    # one thread per worker process: the pool supplies the parallelism
+    from threadpoolctl import threadpool_limits
+    global _POOL_LIMITS
+    _POOL_LIMITS = threadpool_limits(1)
+
+
+def _cpu_worker(job):
+    if job is None:  # start-up task: imports done before the timed map
+        import oracle.sinkhorn  # noqa: F401
+        import scipy.special  # noqa: F401
+        return 0.0, 0
+    x, y, reg, stop, max_iter = job
+    return cpu_baseline_problem(x, y, reg, stop, max_iter)
+
+
+def host_info():
+    """CPU model, core count and thread settings of the host running the CPU leg."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    import scipy
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "usable_cpus": usable,
+            "threads_env": {k: os.environ.get(k) for k in
+                            ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")},
+            "numpy": np.__version__, "scipy": scipy.__version__}
+
+
+def cpu_baseline_family(fam, stop, max_iter, single=24, pooled=96):
+    """The float64 NumPy sinkhorn_log oracle on the first problems of the
+    evaluation set: ``single`` problems in this process (one core), then
+    ``pooled`` problems over a process pool of one worker per usable host CPU
+    (at most 16), one problem per task."""
+    import multiprocessing as mp
+    info = host_info()
+    tot, its = 0.0, 0
+    for q in range(single):
+        dt, nit = cpu_baseline_problem(fam.xs[q], fam.ys[q], fam.regs[q], stop, max_iter)
+        tot += dt
+        its += nit + 1
+    workers = max(1, min(16, info["usable_cpus"] or 1))
+    jobs = [(fam.xs[q], fam.ys[q], fam.regs[q], stop, max_iter) for q in range(pooled)]
+    # fresh interpreters: the workers never touch the GPU this process holds
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(workers, initializer=_cpu_worker_init) as pool:
+        pool.map(_cpu_worker, [None] * workers, chunksize=1)
+        t0 = time.perf_counter()
+        res = pool.map(_cpu_worker, jobs, chunksize=1)
+        wall = time.perf_counter() - t0
+    return {"value": pooled / wall, "unit": "problems/s", "cores": workers, "kind": "port",
+            "sample": f"problems 0-{pooled - 1} of the evaluation set, float64 NumPy "
+                      f"sinkhorn_log (stopThr {stop}), one problem per task over a pool of "
+                      f"{workers} single-threaded processes (started before the timing), "
+                      f"{sum(r[1] + 1 for r in res)} iterations in {wall:.1f} s wall",
+            "single_core": {"value": single / tot, "unit": "problems/s", "cores": 1,
+                            "sample": f"problems 0-{single - 1}, {its} iterations in "
+                                      f"{tot:.1f} s"},
+            "host": info}
+
+
+def dense_c2_figure(torch, dev, steps=2):
+    """One 16384 x 16384 problem (d = 64, Dirichlet weights, reg 1e-2) on its
+    materialised fp32 cost (1 GiB): 200 fixed sinkhorn_log iterations, timed
+    with events on the solve's stream.  Algorithmic bytes: one read of the
+    cost per iteration (the column update is fused into the row sweep), plus
+    the prologue's column pass."""
+    from pot_amd import _lib, _solve, problems
+    x, y, a, b = problems.config2_clouds()
+    n = x.shape[0]
+    X = torch.tensor(x, device=dev)
+    Y = torch.tensor(y, device=dev)
+    M = _solve.sqeuclidean(X[None], Y[None], clamp=True)[0]
+    M /= M.max()
+    at = torch.tensor(a, device=dev).view(1, n)
+    bt = torch.tensor(b, device=dev).view(1, n)
+    z = torch.zeros(1, n, dtype=torch.float64, device=dev)
+    its = 200
+
+    def solve():
+        return _solve.dense_solve(M, at, bt, z, z, 1e-2, domain=_lib.LOG,
+                                  order=_lib.COL_FIRST, max_iter=its, stop_thr=0.0)
+
+    solve()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        solve()
+    e1.record()
+    torch.cuda.synchronize()
+    dt = e0.elapsed_time(e1) * 1e-3 / steps
+    alg = float(n) * n * 4 * (its + 1)
+    gbs = alg / dt / 1e9
+    del M
+    return {"workload": "dense n=m=16384 d=64 Dirichlet weights reg=1e-2, 200 fixed "
+                        "iterations, materialised fp32 cost (1 GiB)",
+            "iterations_per_s": its / dt, "ms_per_iteration": 1e3 * dt / its,
+            "two_pass_equivalent_gbs": 2 * gbs,
+            "roofline": {"bound": "hbm", "kernel": "row_sweep_kernel + col_merge_kernel "
+                         "(whole iteration)", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                         "alg_bytes_per_iteration": float(n) * n * 4,
+                         "traffic": DENSE_C2_TRAFFIC,
+                         "traffic_over_alg": DENSE_C2_TRAFFIC / (float(n) * n * 4)}}
 
 
 def run_warmstart(args, torch, dev):
@@ -169,11 +293,18 @@ def run_warmstart(args, torch, dev):
     traffic = 3.3735e8 * 1024 * 2.000 + 6.610e4 * 1024 * 1.000 \
         if (B == 4096 and args.stop == 1e-6 and args.max_iter == 2000) else None
     if args.cost == "points":
-        # The point-cloud kernel streams no cost: per element of every pass it
-        # issues 4 MFMA products of 32x32x16 (f16 hi.lo, lo.hi, hi.hi and the
-        # bf16 potential slot), 128 flops, and in the fused sweeps one exp2.
+        # Algorithmic work of the point-cloud solve: every iteration forms
+        # x . y^T twice (once per half-step), 2 d flops per element each, so
+        # 4 d = 64 flops per element-iteration at d = 16, counted over
+        # (niter + 1) sweeps of every problem (the last, fused check sweep
+        # included).  The kernel executes more on MFMA: 4 products of
+        # 32x32x16 per element of every pass (f16 hi.lo, lo.hi, hi.hi and the
+        # bf16 potential slot), 128 flops, over niter + 4 passes.
         secs = cold["kernel_ms"] * 1e-3
-        tf = 128.0 * cold["pass_elements"] / secs / 1e12
+        d = 16
+        alg_flops = 4.0 * d * cold["sweep_elements"]
+        tf = alg_flops / secs / 1e12
+        tf_exec = 128.0 * cold["pass_elements"] / secs / 1e12
         # The binding unit is the VALU: per element of a fused sweep one exp2,
         # one add (row sum) and one fma (column sum), measured on this GPU
         # model at 8.2 + 2.6 + 2.7 cycles per wave64 instruction per SIMD
@@ -183,15 +314,19 @@ def run_warmstart(args, torch, dev):
         roof = {"bound": "mfma", "kernel": "batch_points_kernel (cold launch)",
                 "achieved": tf, "peak": MFMA_PEAK_TFLOPS["f16"], "unit": "TFLOP/s",
                 "frac": tf / MFMA_PEAK_TFLOPS["f16"], "traffic": None,
-                "flops_per_launch": 128.0 * cold["pass_elements"],
+                "alg_flops_per_launch": alg_flops,
+                "flops_per_element_iteration": 4 * d,
                 "avg_launch_ms": cold["kernel_ms"],
-                "exp2_per_s": cold["sweep_elements"] / secs,
+                "executed_mfma": {"flops_per_launch": 128.0 * cold["pass_elements"],
+                                  "achieved": tf_exec,
+                                  "frac": tf_exec / MFMA_PEAK_TFLOPS["f16"]},
                 "valu_bound": {"achieved": cold["sweep_elements"] / secs,
                                "peak": valu_peak, "unit": "elements/s",
                                "frac": cold["sweep_elements"] / secs / valu_peak},
-                "note": "executed MFMA flops of the cold launch; the points stay in LDS and "
-                        "registers, so HBM carries only points and potentials; the kernel "
-                        "is bound by the VALU (exp2 + add + fma per element), see valu_bound"}
+                "note": "achieved = algorithmic x.y^T flops (4 d per element-iteration) over "
+                        "the cold launch; the points stay in LDS and registers, so HBM "
+                        "carries only points and potentials (traffic null); the kernel is "
+                        "bound by the VALU (exp2 + add + fma per element), see valu_bound"}
     else:
         roof = {"bound": "hbm", "kernel": "batch_log_kernel (cold launch)",
                 "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -214,18 +349,8 @@ def run_warmstart(args, torch, dev):
         "roofline": roof,
     }
     if not args.no_cpu:
-        # a bounded sample: the first 48 problems of the evaluation set
-        ncpu = 48
-        tot, its = 0.0, 0
-        for q in range(ncpu):
-            dt, nit = cpu_baseline_problem(fam.xs[q], fam.ys[q], fam.regs[q], args.stop,
-                                           args.max_iter)
-            tot += dt
-            its += nit + 1
-        out["cpu_baseline"] = {"value": ncpu / tot, "unit": "problems/s", "cores": 1,
-                               "kind": "port",
-                               "sample": f"problems 0-{ncpu - 1} of the evaluation set, float64 "
-                                         f"NumPy sinkhorn_log, {its} iterations in {tot:.1f} s"}
+        out["cpu_baseline"] = cpu_baseline_family(fam, args.stop, args.max_iter)
+    out["dense_c2"] = dense_c2_figure(torch, dev, steps=2)
     return out
 
 

@@ -71,8 +71,10 @@ def solve_gromov_batch(C1, C2, reg=1e-2, a=None, b=None, symmetric=None, M=None,
     a = np.ones((B, n)) / n if a is None else a
     b = np.ones((B, m)) / m if b is None else b
     if symmetric is None:
-        symmetric = (np.allclose(C1, C1.transpose(0, 2, 1), atol=1e-10)
-                     and np.allclose(C2, C2.transpose(0, 2, 1), atol=1e-10))
+        # transpose of the two structure axes only, channels stay last
+        # (ot/batch/_quadratic.py:575-578)
+        symmetric = (np.allclose(C1, np.swapaxes(C1, 1, 2), atol=1e-10)
+                     and np.allclose(C2, np.swapaxes(C2, 1, 2), atol=1e-10))
     L = tensor_batch(a, b, C1, C2, symmetric, loss=loss, logits=logits)
     if M is None:
         M, alpha = np.zeros((B, n, m)), 1.0

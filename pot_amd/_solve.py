@@ -133,6 +133,15 @@ def gemm(A, Bm, C, *, alpha=1.0, beta=0.0, transA=False, transB=False):
     lib = _lib.load()
     P, Mr, N = C.shape
     K = A.shape[1] if transA else A.shape[2]
+    a_rows = A.shape[2] if transA else A.shape[1]
+    b_k, b_cols = (Bm.shape[2], Bm.shape[1]) if transB else (Bm.shape[1], Bm.shape[2])
+    if (A.dim() != 3 or Bm.dim() != 3 or A.shape[0] != P or Bm.shape[0] != P
+            or a_rows != Mr or b_k != K or b_cols != N or C.stride(2) != 1
+            or not (A.dtype == Bm.dtype == C.dtype)
+            or C.dtype not in (torch.float32, torch.float64)
+            or not (A.device == Bm.device == C.device)):
+        raise ValueError(f"gemm: shapes {tuple(A.shape)} x {tuple(Bm.shape)} -> "
+                         f"{tuple(C.shape)} (transA={transA}, transB={transB}) do not agree")
     sa = (A.stride(2), A.stride(1)) if transA else (A.stride(1), A.stride(2))
     sb = (Bm.stride(2), Bm.stride(1)) if transB else (Bm.stride(1), Bm.stride(2))
     code = _lib.F32 if C.dtype == torch.float32 else _lib.F64

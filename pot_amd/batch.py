@@ -193,6 +193,10 @@ def _pairwise(X, Y, metric, p=2.0, q=1.0, eps=1e-10):
     Y = Y.contiguous()
     B, n, d = X.shape
     m = Y.shape[1]
+    if (Y.dim() != 3 or Y.shape[0] != B or Y.shape[2] != d or X.dtype != Y.dtype
+            or X.dtype not in (torch.float32, torch.float64) or X.device != Y.device):
+        raise ValueError(f"pairwise cost: X {tuple(X.shape)} {X.dtype} and Y "
+                         f"{tuple(Y.shape)} {Y.dtype} do not agree")
     M = torch.empty((B, n, m), dtype=X.dtype, device=X.device)
     code = _lib.F32 if X.dtype == torch.float32 else _lib.F64
     rc = lib.potamd_pairwise_batch(

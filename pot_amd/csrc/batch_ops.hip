@@ -74,10 +74,14 @@ __global__ __launch_bounds__(256) void pairwise_kernel(const T* __restrict__ X,
         for (int r = 0; r < 4; ++r)
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
-            if (METRIC == 0)
+            if (METRIC == 0) {
               acc[r][c] += abs_pow(xv[r] - yv[c], p);
-            else
-              acc[r][c] = fma(yv[c], xv[r], acc[r][c]);
+            } else {
+              // rounded product, then the add, in the order of the entropy
+              // sum: KL(y || x) of a row with itself is then exactly 0
+#pragma clang fp contract(off)
+              acc[r][c] = acc[r][c] + yv[c] * xv[r];
+            }
           }
       }
       __syncthreads();
