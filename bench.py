@@ -72,6 +72,8 @@ def _args():
                     help="ragged: share large problems among workgroups when the batch is small")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cold-only", action="store_true")
+    ap.add_argument("--no-c2", action="store_true",
+                    help="warmstart: skip the 16384^2 dense figure (profiling runs)")
     return ap.parse_args()
 
 
@@ -350,7 +352,8 @@ def run_warmstart(args, torch, dev):
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_family(fam, args.stop, args.max_iter)
-    out["dense_c2"] = dense_c2_figure(torch, dev, steps=2)
+    if not args.no_c2:
+        out["dense_c2"] = dense_c2_figure(torch, dev, steps=2)
     return out
 
 

@@ -89,6 +89,27 @@ int potamd_sinkhorn_dense(int dtype, int domain, int order, int kernel_given,
                           int prefetch, void* workspace, size_t ws_bytes, void* host_ctrl,
                           void* stream, int* info);
 
+/* The same dense solve with each problem stopping on its own check, as a
+ * lone call per problem would (a batch of sinkhorn_log problems, e.g. the
+ * histograms of a 2-D b sharing M through bstride = 0).  Replaces the
+ * per-column loop of the multi-histogram sinkhorn_log
+ * (ot/bregman/_sinkhorn.py:816-854: one sinkhorn_log call per column of b,
+ * each with its own stop at :884-899) with one batched solve.
+ * niters, statuses: device int[batch], each problem's `ii` at its stop and
+ * its status (POTAMD_CONVERGED or POTAMD_MAX_ITER).  u, v receive every
+ * problem's state at its own stop.  The return value is POTAMD_CONVERGED
+ * when every problem converged, POTAMD_MAX_ITER when one reached max_iter,
+ * POTAMD_NEED_EXACT as for potamd_sinkhorn_dense (then u, v are not written).
+ * Workspace: potamd_dense_workspace_size.  No error history is kept.
+ */
+int potamd_sinkhorn_dense_each(int dtype, int domain, int order, int kernel_given,
+                               const void* M, int64_t ldm, int64_t bstride, int batch, int n,
+                               int m, double reg, const double* a, const double* b, double* u,
+                               double* v, int max_iter, double stop_thr, int threads,
+                               int cols_per_thread, int rows_per_block, int prefetch,
+                               void* workspace, size_t ws_bytes, void* host_ctrl, void* stream,
+                               int* niters, int* statuses, int* info);
+
 /* Bytes of workspace needed by potamd_plan_dense. */
 size_t potamd_plan_workspace_size(int batch, int n);
 
@@ -402,6 +423,28 @@ int potamd_sinkhorn_stabilized(int dtype, const void* M, int64_t ldm, int n, int
                                int print_period, double* err_hist, int err_cap, double* marg_sq,
                                void* workspace, size_t ws_bytes, void* host_ctrl, void* stream,
                                int* info);
+
+/* The several-histogram branch of ot.bregman.sinkhorn_stabilized (a 2-D b,
+ * ot/bregman/_sinkhorn.py:1230-1367): H target histograms against one M.
+ * b: H x m histogram-major; u: H x n, v: H x m (out: the final scalings);
+ * alpha (n), beta (m): in the warm start, out the absorbed potentials.
+ * Per iteration v = b / (K^T u), u = a / (K v) for every histogram; when
+ * max|u| or max|v| exceeds tau, alpha += reg max_h log u_h (row-wise),
+ * beta += reg max log v (one maximum over all of v) and K is rebuilt, with
+ * u and v kept (the reference resets them only for one histogram); every
+ * print_period-th iteration err = (max|u - u'| / max(max|u|, max|u'|, 1) +
+ * the same for v) / 2 is logged; err <= stop_thr stops; a NaN restores the
+ * previous u, v and stops (status POTAMD_NUMERICAL).  info as for
+ * potamd_sinkhorn_stabilized.  Workspace:
+ * potamd_stabilized_multi_workspace_size.
+ */
+size_t potamd_stabilized_multi_workspace_size(int dtype, int n, int m, int H);
+int potamd_sinkhorn_stabilized_multi(int dtype, const void* M, int64_t ldm, int n, int m, int H,
+                                     double reg, const double* a, const double* b,
+                                     double* alpha, double* beta, double* u, double* v,
+                                     double tau, int max_iter, double stop_thr, int print_period,
+                                     double* err_hist, int err_cap, void* workspace,
+                                     size_t ws_bytes, void* host_ctrl, void* stream, int* info);
 
 #ifdef __cplusplus
 }

@@ -12,12 +12,20 @@ log n_iter, logu, logv, alpha, beta, warmstart) and :1371-1566 (epsilon
 scaling: reg_i = (epsilon0 - reg) e^-i + reg, inner stabilized solves warm
 started by (alpha, beta), err = ||G^T 1 - b||^2 + ||G 1 - a||^2 every
 print_period-th outer iteration, stop when err <= stopThr after 35 outer
-iterations).  Single histograms only.
+iterations).  ``sinkhorn_stabilized_multi`` is the several-histogram branch
+(a 2-D b, :1230-1367): scalings (n, H) and (m, H) starting at 1/dim,
+absorption ``alpha += reg max_h log u``, ``beta += reg max log v`` without
+resetting u, v, error ``(max|u - u'| / max(max|u|, max|u'|, 1) + the same
+for v) / 2`` every print_period-th iteration, per-histogram losses
+``sum(Gamma_h * M)`` and (n, H) / (m, H) logs.
 
 Pinned (tests/test_oracle.py) by the doctests (_sinkhorn.py:1191-1193,
 :1458-1460), by stabilized == sinkhorn_log at convergence (test_bregman.py:
 282-314), and by absorption invariance (tau small enough to absorb at every
-iteration gives the same plan).
+iteration gives the same plan).  The several-histogram branch is pinned by
+its fixed point: at a tight tolerance each histogram's loss equals the
+single-histogram loss of the same column, with and without absorption, and
+identical columns give identical scalings.
 """
 
 import warnings
@@ -83,6 +91,69 @@ def sinkhorn_stabilized(a, b, M, reg, numItermax=1000, tau=1e3, stopThr=1e-9, wa
                 "beta": beta + reg * np.log(v)}
     logd["warmstart"] = (logd["alpha"], logd["beta"])
     return (G, logd) if log else G
+
+
+def sinkhorn_stabilized_multi(a, b, M, reg, numItermax=1000, tau=1e3, stopThr=1e-9,
+                              warmstart=None, print_period=20, log=False, warn=True):
+    """The several-histogram branch of _sinkhorn.py:1230-1367 (b: (m, H))."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    M = np.asarray(M, dtype=np.float64)
+    dim_a, dim_b = M.shape
+    if len(a) == 0:
+        a = np.full(dim_a, 1.0 / dim_a)
+    H = b.shape[1]
+    a2 = a[:, None]
+    errs = []
+    if warmstart is None:
+        alpha, beta = np.zeros(dim_a), np.zeros(dim_b)
+    else:
+        alpha, beta = (np.asarray(w, dtype=np.float64).copy() for w in warmstart)
+    u = np.full((dim_a, H), 1.0 / dim_a)
+    v = np.full((dim_b, H), 1.0 / dim_b)
+
+    def get_K(al, be):
+        return np.exp(-(M - al[:, None] - be[None, :]) / reg)
+
+    K = get_K(alpha, beta)
+    err = 1
+    ii = -1
+    with np.errstate(all="ignore"):
+        for ii in range(numItermax):
+            uprev, vprev = u, v
+            v = b / (K.T @ u)
+            u = a2 / (K @ v)
+            if np.max(np.abs(u)) > tau or np.max(np.abs(v)) > tau:
+                # the reference resets u, v only in its one-histogram branch
+                alpha = alpha + reg * np.max(np.log(u), 1)
+                beta = beta + reg * np.max(np.log(v))
+                K = get_K(alpha, beta)
+            if ii % print_period == 0:
+                err_u = np.max(np.abs(u - uprev))
+                err_u /= max(np.max(np.abs(u)), np.max(np.abs(uprev)), 1.0)
+                err_v = np.max(np.abs(v - vprev))
+                err_v /= max(np.max(np.abs(v)), np.max(np.abs(vprev)), 1.0)
+                err = 0.5 * (err_u + err_v)
+                errs.append(err)
+            if err <= stopThr:
+                break
+            if np.any(np.isnan(u)) or np.any(np.isnan(v)):
+                warnings.warn("Numerical errors at iteration %d" % ii)
+                u, v = uprev, vprev
+                break
+        else:
+            if warn:
+                warnings.warn("Sinkhorn did not converge. You might want to "
+                              "increase the number of iterations `numItermax` "
+                              "or the regularization parameter `reg`.")
+        al, be = alpha[:, None], beta[:, None]
+        losses = np.array([np.sum(np.exp(-(M - al - be.T) / reg + np.log(u[:, [h]])
+                                         + np.log(v[:, [h]]).T) * M) for h in range(H)])
+        logd = {"err": errs, "n_iter": ii, "logu": al / reg + np.log(u),
+                "logv": be / reg + np.log(v), "alpha": al + reg * np.log(u),
+                "beta": be + reg * np.log(v)}
+    logd["warmstart"] = (logd["alpha"], logd["beta"])
+    return (losses, logd) if log else losses
 
 
 def sinkhorn_epsilon_scaling(a, b, M, reg, numItermax=100, epsilon0=1e4, numInnerItermax=100,

@@ -30,6 +30,14 @@ SIGNATURES = {
         c_int, c_dbl, c_int, c_vp, c_int,    # max_iter, stop_thr, err_norm, err_hist, err_cap
         c_int, c_int, c_int, c_int,          # threads, cols_per_thread, rows_per_block, prefetch
         c_vp, c_size, c_vp, c_vp, c_ip]),    # workspace, ws_bytes, host_ctrl, stream, info
+    "potamd_sinkhorn_dense_each": (c_int, [
+        c_int, c_int, c_int, c_int,          # dtype, domain, order, kernel_given
+        c_vp, c_i64, c_i64, c_int, c_int, c_int,  # M, ldm, bstride, batch, n, m
+        c_dbl, c_vp, c_vp, c_vp, c_vp,       # reg, a, b, u, v
+        c_int, c_dbl,                        # max_iter, stop_thr
+        c_int, c_int, c_int, c_int,          # threads, cols_per_thread, rows_per_block, prefetch
+        c_vp, c_size, c_vp, c_vp,            # workspace, ws_bytes, host_ctrl, stream
+        c_vp, c_vp, c_ip]),                  # niters, statuses, info
     "potamd_plan_workspace_size": (c_size, [c_int, c_int]),
     "potamd_plan_dense": (c_int, [
         c_int, c_int, c_int, c_vp, c_i64, c_i64, c_int, c_int, c_int, c_dbl,
@@ -91,6 +99,13 @@ SIGNATURES = {
         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,        # a, b, alpha, beta, u, v
         c_dbl, c_int, c_dbl, c_int,                # tau, max_iter, stop_thr, print_period
         c_vp, c_int, c_vp,                         # err_hist, err_cap, marg_sq
+        c_vp, c_size, c_vp, c_vp, c_ip]),          # workspace, ws_bytes, host_ctrl, stream, info
+    "potamd_stabilized_multi_workspace_size": (c_size, [c_int, c_int, c_int, c_int]),
+    "potamd_sinkhorn_stabilized_multi": (c_int, [
+        c_int, c_vp, c_i64, c_int, c_int, c_int, c_dbl,  # dtype, M, ldm, n, m, H, reg
+        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,        # a, b (H x m), alpha, beta, u, v
+        c_dbl, c_int, c_dbl, c_int,                # tau, max_iter, stop_thr, print_period
+        c_vp, c_int,                               # err_hist, err_cap
         c_vp, c_size, c_vp, c_vp, c_ip]),          # workspace, ws_bytes, host_ctrl, stream, info
     "potamd_pairwise_batch": (c_int, [
         c_int, c_int, c_vp, c_vp, c_int, c_int, c_int, c_int,  # dtype, metric, X, Y, B, n, m, d

@@ -20,12 +20,16 @@ class DenseResult:
     errs: list             # marginal errors of the check iterations
     status: int            # _lib.CONVERGED / MAX_ITER / NUMERICAL
     bad_iter: int
+    niters: torch.Tensor = None    # (B,) int32 per-problem stops (each=True only)
+    statuses: torch.Tensor = None  # (B,) int32 per-problem statuses (each=True only)
 
 
 def dense_solve(M, a, b, u0, v0, reg, *, domain, order, kernel_given=False,
                 max_iter=1000, stop_thr=1e-9, err_norm=0, batch=None,
-                bstride=None, tile=None):
-    """Run potamd_sinkhorn_dense.
+                bstride=None, tile=None, each=False):
+    """Run potamd_sinkhorn_dense (or, with ``each``, potamd_sinkhorn_dense_each:
+    every problem stops on its own check, and the result's ``niters`` /
+    ``statuses`` hold the per-problem stops).
 
     M: (n, m) or (B, n, m) device tensor (float32/float64), rows contiguous.
     a: (B, n), b: (B, m), u0: (B, n), v0: (B, m) float64 device tensors.
@@ -59,22 +63,36 @@ def dense_solve(M, a, b, u0, v0, reg, *, domain, order, kernel_given=False,
     err_hist = torch.zeros(err_cap, dtype=torch.float64, device=dev)
     info = (ctypes.c_int * 4)()
     hc = _device.host_ctrl()
-    rc = lib.potamd_sinkhorn_dense(
-        code, domain, order, int(bool(kernel_given)),
-        _ptr(M), ldm, bs, B, n, m, float(reg) if reg is not None else 1.0,
-        _ptr(a), _ptr(b), _ptr(u), _ptr(v),
-        int(max_iter), float(stop_thr), int(err_norm), _ptr(err_hist), err_cap,
-        T, cpt, rpb, pf, _ptr(ws), ws_bytes, ctypes.c_void_p(hc.data_ptr()),
-        ctypes.c_void_p(_device.stream_ptr(dev)), info)
-    _lib.check(rc, "potamd_sinkhorn_dense")
+    if a.numel() != B * n or b.numel() != B * m or u.numel() != B * n or v.numel() != B * m:
+        raise ValueError("weights and potentials must hold B x n and B x m values")
+    niters = statuses = None
+    if each:
+        niters = torch.full((B,), -1, dtype=torch.int32, device=dev)
+        statuses = torch.full((B,), -1, dtype=torch.int32, device=dev)
+        rc = lib.potamd_sinkhorn_dense_each(
+            code, domain, order, int(bool(kernel_given)),
+            _ptr(M), ldm, bs, B, n, m, float(reg) if reg is not None else 1.0,
+            _ptr(a), _ptr(b), _ptr(u), _ptr(v), int(max_iter), float(stop_thr),
+            T, cpt, rpb, pf, _ptr(ws), ws_bytes, ctypes.c_void_p(hc.data_ptr()),
+            ctypes.c_void_p(_device.stream_ptr(dev)), _ptr(niters), _ptr(statuses), info)
+        _lib.check(rc, "potamd_sinkhorn_dense_each")
+    else:
+        rc = lib.potamd_sinkhorn_dense(
+            code, domain, order, int(bool(kernel_given)),
+            _ptr(M), ldm, bs, B, n, m, float(reg) if reg is not None else 1.0,
+            _ptr(a), _ptr(b), _ptr(u), _ptr(v),
+            int(max_iter), float(stop_thr), int(err_norm), _ptr(err_hist), err_cap,
+            T, cpt, rpb, pf, _ptr(ws), ws_bytes, ctypes.c_void_p(hc.data_ptr()),
+            ctypes.c_void_p(_device.stream_ptr(dev)), info)
+        _lib.check(rc, "potamd_sinkhorn_dense")
     if rc == _lib.NEED_EXACT:
         raise _lib.LibraryError(
             "a column mass fell below the fp32 range at iteration %d; "
             "use float64 inputs" % info[0])
-    n_err = min(info[1], err_cap)
+    n_err = 0 if each else min(info[1], err_cap)
     errs = err_hist[:n_err].cpu().tolist() if n_err else []
     return DenseResult(u=u.view(B, n), v=v.view(B, m), niter=info[0], errs=errs,
-                       status=info[2], bad_iter=info[3])
+                       status=info[2], bad_iter=info[3], niters=niters, statuses=statuses)
 
 
 def plan(M, u, v, reg, *, domain, kernel_given=False, a=None, b=None,

@@ -90,9 +90,6 @@ def sinkhorn2(a, b, M, reg, method="sinkhorn", numItermax=1000, stopThr=1e-9,
     M, a, b = list_to_array(M, a, b)
     name = method.lower()
     if name == "sinkhorn_stabilized":
-        if len(b.shape) >= 2:
-            raise NotImplementedError("sinkhorn_stabilized with several histograms is not "
-                                      "part of this build")
         from .stabilized import sinkhorn_stabilized
         return sinkhorn_stabilized(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
                                    warmstart=warmstart, verbose=verbose, log=log, warn=warn,
@@ -142,10 +139,16 @@ def sinkhorn_log(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
     ing, Mt, at, bt = _prep(a, b, M)
     n, m = Mt.shape
     if bt.dim() > 1:
-        # Multi-histogram b: one solve per column, warn not forwarded
-        # (_sinkhorn.py:816-854).
         nh = bt.shape[1]
+        if nh == 1 and warmstart is None:
+            # the reference builds per-column warm starts only for n_hists > 1
+            # and then indexes None (_sinkhorn.py:816, :834)
+            raise TypeError("'NoneType' object is not subscriptable")
         ws = warmstart if warmstart is not None else [None] * nh
+        if numItermax > 0 and not verbose:
+            return _sinkhorn_log_columns(ing, Mt, at, bt, reg, numItermax, stopThr, log, ws)
+        # Multi-histogram b, verbose or numItermax <= 0: one solve per column,
+        # warn not forwarded (_sinkhorn.py:816-854).
         b_np = b if not isinstance(b, list) else np.asarray(b)
         losses, lus, lvs = [], [], []
         for k in range(nh):
@@ -216,6 +219,39 @@ def sinkhorn_log(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
     if log:
         return out, logd
     return out
+
+
+def _sinkhorn_log_columns(ing, Mt, at, bt, reg, numItermax, stopThr, log, ws):
+    """Multi-histogram ``sinkhorn_log`` (_sinkhorn.py:816-854) in one launch:
+    the histograms are a batch sharing M (bstride 0), and each stops on its
+    own check exactly as the reference's per-column call does, so potentials
+    and losses equal those of the column loop.  Each column that reaches
+    numItermax warns, as the reference's per-column calls do (their ``warn``
+    is left at its default)."""
+    n, m = Mt.shape
+    nh = bt.shape[1]
+    u0 = torch.zeros((nh, n), dtype=torch.float64, device=ing.device)
+    v0 = torch.zeros((nh, m), dtype=torch.float64, device=ing.device)
+    for k in range(nh):
+        if ws[k] is not None:
+            u0[k] = ing.dev(ws[k][0], torch.float64).reshape(n)
+            v0[k] = ing.dev(ws[k][1], torch.float64).reshape(m)
+    a_b = at.view(1, n).expand(nh, n).contiguous()
+    b_b = bt.t().contiguous()
+    res = _solve.dense_solve(Mt, a_b, b_b, u0, v0, reg, domain=_lib.LOG, order=_lib.COL_FIRST,
+                             max_iter=numItermax, stop_thr=stopThr, batch=nh, bstride=0,
+                             each=True)
+    for st in res.statuses.cpu().tolist():
+        if st == _lib.MAX_ITER:
+            warnings.warn(NOT_CONVERGED)
+    _, _, sums = _solve.plan(Mt, res.u, res.v, reg, domain=_lib.LOG, want_plan=False,
+                             batch=nh, bstride=0)
+    out = ing.out(sums[:, 0])
+    if not log:
+        return out
+    lu, lv = res.u.t(), res.v.t()
+    return out, {"log_u": ing.out(lu), "log_v": ing.out(lv), "u": ing.out(torch.exp(lu)),
+                 "v": ing.out(torch.exp(lv))}
 
 
 def sinkhorn_knopp(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,

@@ -55,6 +55,7 @@ struct DenseLayout {
   double* psum;
   Ctrl* ctrl;
   int* bad;
+  int* pstate;  // per-problem stop: [B] done, [B] niter, [B] status
   size_t bytes;
 };
 
@@ -77,6 +78,7 @@ DenseLayout carve_dense(void* ws, int dtype, int B, int n, int m, int rpb) {
   L.psum = c.take<double>((size_t)B * G2 * m);
   L.ctrl = c.take<Ctrl>(1);
   L.bad = c.take<int>(2);
+  L.pstate = c.take<int>((size_t)3 * B);
   L.bytes = c.off;
   return L;
 }
@@ -111,20 +113,25 @@ extern "C" size_t potamd_dense_workspace_size(int dtype, int batch, int n, int m
   return carve_dense(nullptr, dtype, batch, n, m, rpb).bytes;
 }
 
-extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kernel_given,
-                                     const void* M, int64_t ldm, int64_t bstride, int batch,
-                                     int n, int m, double reg, const double* a, const double* b,
-                                     double* u, double* v, int max_iter, double stop_thr,
-                                     int err_norm, double* err_hist, int err_cap, int threads,
-                                     int cols_per_thread, int rows_per_block, int prefetch,
-                                     void* workspace,
-                                     size_t ws_bytes, void* host_ctrl, void* stream, int* info) {
+namespace {
+
+// The dense solve behind potamd_sinkhorn_dense (err_norm 0 or 1: one stop
+// for the batch) and potamd_sinkhorn_dense_each (err_norm 2: each problem
+// stops on its own check; niters / statuses receive the per-problem results).
+int dense_impl(int dtype, int domain, int order, int kernel_given, const void* M, int64_t ldm,
+               int64_t bstride, int batch, int n, int m, double reg, const double* a,
+               const double* b, double* u, double* v, int max_iter, double stop_thr,
+               int err_norm, double* err_hist, int err_cap, int threads, int cols_per_thread,
+               int rows_per_block, int prefetch, void* workspace, size_t ws_bytes,
+               void* host_ctrl, void* stream, int* info, int* niters, int* statuses) {
   if ((dtype != POTAMD_F32 && dtype != POTAMD_F64) || (domain != POTAMD_LOG && domain != POTAMD_SCALING) ||
       (order != POTAMD_COL_FIRST && order != POTAMD_ROW_FIRST) || !M || !a || !b || !u || !v ||
-      !info || batch < 1 || n < 1 || m < 1 || ldm < m || bstride < 0 || (err_norm != 0 && err_norm != 1) ||
+      !info || batch < 1 || n < 1 || m < 1 || ldm < m || bstride < 0 ||
+      err_norm < 0 || err_norm > 2 ||
       max_iter < 1 || (!kernel_given && !(reg > 0.0) && domain == POTAMD_LOG) ||
       (err_cap > 0 && !err_hist))
     return POTAMD_EINVAL;
+  const bool each = err_norm == 2;
   const int rpb = rows_per_block > 0 ? rows_per_block : default_rpb(n);
   if (rpb > 256) return POTAMD_EINVAL;
   int T = threads, cpt = cols_per_thread, pf = prefetch;
@@ -159,6 +166,7 @@ extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kerne
 
   // Set-up: log2 weights, initial state, control block.
   HIP_TRY(launch_ctrl_init(L.ctrl, L.bad, s));
+  if (each) HIP_TRY(launch_pstate_init(L.pstate, batch, s));
   HIP_TRY(launch_log2(Bn, a, L.loga2, s));
   HIP_TRY(launch_log2(Bm, b, L.logb2, s));
   HIP_TRY(launch_scale_copy(Bn, u, logd ? kLog2e : 1.0, L.U[0], s));
@@ -183,6 +191,7 @@ extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kerne
   ra.rowerr = L.rowerr;
   ra.bad = L.bad;
   ra.ctrl = L.ctrl;
+  ra.pdone = each ? L.pstate : nullptr;
 
   MergeArgs ma;
   memset(&ma, 0, sizeof(ma));
@@ -204,6 +213,7 @@ extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kerne
   ma.err_cap = err_cap;
   ma.stop_thr = stop_thr;
   ma.err_norm = err_norm;
+  ma.pstate = each ? L.pstate : nullptr;
 
   // Prologue of the column-first order: v_1 from u_0.
   if (order == POTAMD_COL_FIRST) {
@@ -310,14 +320,51 @@ extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kerne
     fv = (niter + 1) & 1;
   }
   if (status != POTAMD_NEED_EXACT) {
-    HIP_TRY(launch_scale_copy(Bn, L.U[fu], logd ? kLn2 : 1.0, u, s));
-    HIP_TRY(launch_scale_copy(Bm, L.V[fv], logd ? kLn2 : 1.0, v, s));
+    if (each) {
+      HIP_TRY(launch_gather_final(batch, n, m, L.pstate, L.U, L.V, logd ? kLn2 : 1.0, u, v,
+                                  niters, statuses, s));
+    } else {
+      HIP_TRY(launch_scale_copy(Bn, L.U[fu], logd ? kLn2 : 1.0, u, s));
+      HIP_TRY(launch_scale_copy(Bm, L.V[fv], logd ? kLn2 : 1.0, v, s));
+    }
   }
   info[0] = niter;
   info[1] = local.n_err;
   info[2] = status;
   info[3] = local.bad_iter;
   return status;
+}
+
+}  // namespace
+
+extern "C" int potamd_sinkhorn_dense(int dtype, int domain, int order, int kernel_given,
+                                     const void* M, int64_t ldm, int64_t bstride, int batch,
+                                     int n, int m, double reg, const double* a, const double* b,
+                                     double* u, double* v, int max_iter, double stop_thr,
+                                     int err_norm, double* err_hist, int err_cap, int threads,
+                                     int cols_per_thread, int rows_per_block, int prefetch,
+                                     void* workspace,
+                                     size_t ws_bytes, void* host_ctrl, void* stream, int* info) {
+  if (err_norm != 0 && err_norm != 1) return POTAMD_EINVAL;
+  return dense_impl(dtype, domain, order, kernel_given, M, ldm, bstride, batch, n, m, reg, a, b,
+                    u, v, max_iter, stop_thr, err_norm, err_hist, err_cap, threads,
+                    cols_per_thread, rows_per_block, prefetch, workspace, ws_bytes, host_ctrl,
+                    stream, info, nullptr, nullptr);
+}
+
+extern "C" int potamd_sinkhorn_dense_each(int dtype, int domain, int order, int kernel_given,
+                                          const void* M, int64_t ldm, int64_t bstride,
+                                          int batch, int n, int m, double reg, const double* a,
+                                          const double* b, double* u, double* v, int max_iter,
+                                          double stop_thr, int threads, int cols_per_thread,
+                                          int rows_per_block, int prefetch, void* workspace,
+                                          size_t ws_bytes, void* host_ctrl, void* stream,
+                                          int* niters, int* statuses, int* info) {
+  if (!niters || !statuses) return POTAMD_EINVAL;
+  return dense_impl(dtype, domain, order, kernel_given, M, ldm, bstride, batch, n, m, reg, a, b,
+                    u, v, max_iter, stop_thr, 2, nullptr, 0, threads, cols_per_thread,
+                    rows_per_block, prefetch, workspace, ws_bytes, host_ctrl, stream, info,
+                    niters, statuses);
 }
 
 extern "C" size_t potamd_plan_workspace_size(int batch, int n) {

@@ -85,6 +85,7 @@ __global__ __launch_bounds__(T) void row_sweep_kernel(RowSweepArgs A) {
   constexpr int NCH = CPT / VEC;  // vector chunks per thread
   if (A.ctrl->done) return;
   const int p = blockIdx.y;
+  if (A.pdone && A.pdone[p]) return;  // this problem has stopped on its own
   const int g = blockIdx.x;
   const int G = gridDim.x;
   const int tid = threadIdx.x;
@@ -338,6 +339,7 @@ __global__ __launch_bounds__(256) void col_merge_kernel(MergeArgs A) {
   constexpr int NG = 256 / NQ;        // groups of row blocks
   __shared__ double gsum[NG][kMergeCols];
   const int p = blockIdx.y;
+  if (A.pstate && A.pstate[p]) return;  // this problem has stopped on its own
   const int tid = threadIdx.x;
   const int m = A.m;
   const bool col_first = A.order == kColFirst;
@@ -439,6 +441,62 @@ __global__ __launch_bounds__(256) void col_decide_kernel(MergeArgs A) {
   const bool check_col = col_first && (it % 10 == 0);
   const bool check_row = !col_first && it >= 1 && ((it - 1) % 10 == 0);
   const int nblk = (A.m + kMergeCols - 1) / kMergeCols;
+  if (A.err_norm == 2) {
+    // Each problem stops on its own check, exactly as a lone solve would:
+    // a wave per problem (problems w, w + 4, ...), then the solve stops when
+    // none is left running or at the last iteration.
+    __shared__ int running;
+    __shared__ int need_exact;
+    if (tid == 0) {
+      running = 0;
+      need_exact = ctrl->need_exact;
+    }
+    __syncthreads();
+    if (need_exact) {
+      if (tid == 0) {
+        ctrl->done = 1;
+        ctrl->status = kNeedExact;
+        ctrl->niter = it;
+      }
+      return;
+    }
+    int* pdone = A.pstate;
+    int* pniter = A.pstate + A.B;
+    int* pstat = A.pstate + 2 * A.B;
+    const bool last = it >= A.max_iter - 1;
+    for (int q = w; q < A.B; q += 4) {
+      if (pdone[q]) continue;  // uniform per wave
+      bool stop = false;
+      if (check_col || check_row) {
+        const double* src = check_col ? A.errpart + (int64_t)q * nblk : A.rowerr + (int64_t)q * A.G;
+        const int cnt = check_col ? nblk : A.G;
+        double acc = 0.0;
+        for (int b = lane; b < cnt; b += 64) acc += src[b];
+        acc = wave_sum(acc);
+        stop = sqrt(acc) < A.stop_thr;
+      }
+      if (lane == 0) {
+        if (stop) {
+          pdone[q] = 1;
+          pniter[q] = check_col ? it : it - 1;
+          pstat[q] = kConverged;
+        } else if (last) {
+          pdone[q] = 1;
+          pniter[q] = it;
+          pstat[q] = kMaxIter;
+        } else {
+          atomicAdd(&running, 1);
+        }
+      }
+    }
+    __syncthreads();
+    if (tid == 0 && running == 0) {
+      ctrl->done = 1;
+      ctrl->status = last ? kMaxIter : kConverged;
+      ctrl->niter = it;
+    }
+    return;
+  }
   __shared__ double wpart[4];
   if (check_col || check_row) {
     double tot = 0.0;
@@ -565,6 +623,32 @@ __global__ void scale_copy_kernel(int64_t count, const double* x, double f, doub
 __global__ void fill_kernel(int64_t count, double val, double* y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) y[i] = val;
+}
+__global__ void pstate_init_kernel(int* pstate, int B) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < B) {
+    pstate[i] = 0;
+    pstate[B + i] = -1;
+    pstate[2 * B + i] = kMaxIter;
+  }
+}
+// Final potentials of a per-problem solve: problem p's state sits in the
+// ping-pong buffer of its own stop, (niter_p + 1) & 1.
+__global__ void gather_final_kernel(int B, int n, int m, const int* pstate, const double* U0,
+                                    const double* U1, const double* V0, const double* V1,
+                                    double f, double* u, double* v, int* niters, int* statuses) {
+  const int p = blockIdx.y;
+  const int slot = (pstate[B + p] + 1) & 1;
+  const double* Us = slot ? U1 : U0;
+  const double* Vs = slot ? V1 : V0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    u[(int64_t)p * n + i] = Us[(int64_t)p * n + i] * f;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x)
+    v[(int64_t)p * m + j] = Vs[(int64_t)p * m + j] * f;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (niters) niters[p] = pstate[B + p];
+    if (statuses) statuses[p] = pstate[2 * B + p];
+  }
 }
 __global__ void ctrl_init_kernel(Ctrl* c, int* bad) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -717,6 +801,21 @@ hipError_t launch_log2(int64_t count, const double* x, double* y, hipStream_t s)
   hipLaunchKernelGGL(log2_kernel, dim3(nblocks(count, 256)), dim3(256), 0, s, count, x, y);
   return hipGetLastError();
 }
+hipError_t launch_pstate_init(int* pstate, int B, hipStream_t s) {
+  hipLaunchKernelGGL(pstate_init_kernel, dim3((B + 255) / 256), dim3(256), 0, s, pstate, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_final(int B, int n, int m, const int* pstate, double* const* U,
+                               double* const* V, double f, double* u, double* v, int* niters,
+                               int* statuses, hipStream_t s) {
+  const int mx = n > m ? n : m;
+  const int gx = (mx + 255) / 256 < 64 ? (mx + 255) / 256 : 64;
+  hipLaunchKernelGGL(gather_final_kernel, dim3(gx, B), dim3(256), 0, s, B, n, m, pstate, U[0],
+                     U[1], V[0], V[1], f, u, v, niters, statuses);
+  return hipGetLastError();
+}
+
 hipError_t launch_scale_copy(int64_t count, const double* x, double f, double* y, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(scale_copy_kernel, dim3(nblocks(count, 256)), dim3(256), 0, s, count, x, f,

@@ -24,6 +24,7 @@ struct RowSweepArgs {
   double* rowerr;       // B x G partial row-marginal err^2
   int* bad;             // [2] numerical-guard slots (scaling domain)
   const Ctrl* ctrl;
+  const int* pdone;     // per-problem stop: pdone[p] != 0 skips problem p (or null)
 };
 
 struct MergeArgs {
@@ -43,7 +44,9 @@ struct MergeArgs {
   int err_cap;
   double stop_thr;
   int prologue;         // scaling prologue: v1 = b / (K^T u0), no check
-  int err_norm;         // batch error: 0 max over problems, 1 Frobenius
+  int err_norm;         // batch error: 0 max over problems, 1 Frobenius, 2 each
+                        // problem stops on its own error (per-problem state in pstate)
+  int* pstate;          // err_norm 2: [B] done, [B] niter, [B] status
 };
 
 struct ColLseArgs {
@@ -205,6 +208,10 @@ hipError_t launch_log2(int64_t count, const double* x, double* y, hipStream_t s)
 hipError_t launch_scale_copy(int64_t count, const double* x, double f, double* y, hipStream_t s);
 hipError_t launch_fill(int64_t count, double val, double* y, hipStream_t s);
 hipError_t launch_ctrl_init(Ctrl* c, int* bad, hipStream_t s);
+hipError_t launch_pstate_init(int* pstate, int B, hipStream_t s);
+hipError_t launch_gather_final(int B, int n, int m, const int* pstate, double* const* U,
+                               double* const* V, double f, double* u, double* v,
+                               int* niters, int* statuses, hipStream_t s);
 
 // Point-cloud batch solver (batch_points.hip).  The split exchange slots
 // are kPointsSplitSlot bytes each; a slot's counters live in its first

@@ -1,5 +1,7 @@
 // Log-stabilised Sinkhorn (ot.bregman.sinkhorn_stabilized,
-// ot/bregman/_sinkhorn.py:1105-1367) on a materialised cost.
+// ot/bregman/_sinkhorn.py:1105-1367) on a materialised cost, for one target
+// histogram or several (a 2-D b: the scalings become n x H and m x H, the
+// absorption and the error follow the reference's several-histogram branch).
 //
 // Per iteration, exactly as the reference:
 //   v = b / (K^T u);  u = a / (K v)                          (:1268-1270)
@@ -56,6 +58,7 @@ __global__ __launch_bounds__(256) void stab_build_k(const TM* M, int64_t ldm, in
 }
 
 // part[g][j] = sum_{i in block g} K_ij w_i  (fp64)
+// (histogram h = blockIdx.z: w + h n, part + h G m)
 template <typename TM>
 __global__ __launch_bounds__(256) void stab_col_partial(const TM* K, int n, int m,
                                                         const double* w, double* part,
@@ -63,11 +66,13 @@ __global__ __launch_bounds__(256) void stab_col_partial(const TM* K, int n, int 
   if (sc && sc->c.done) return;
   const int j = blockIdx.x * 256 + threadIdx.x;
   const int g = blockIdx.y;
+  const int h = blockIdx.z;
   if (j >= m) return;
+  w += (int64_t)h * n;
   const int r0 = g * kColRows, r1 = min(n, r0 + kColRows);
   double acc = 0.0;
   for (int i = r0; i < r1; ++i) acc = fma((double)K[(int64_t)i * m + j], w[i], acc);
-  part[(int64_t)g * m + j] = acc;
+  part[((int64_t)h * gridDim.y + g) * m + j] = acc;
 }
 
 // v_j = b_j / sum_g part[g][j]
@@ -76,10 +81,12 @@ __global__ __launch_bounds__(256) void stab_col_update(const double* part, int G
                                                        const StabCtrl* sc) {
   if (sc->c.done) return;
   const int j = blockIdx.x * 256 + threadIdx.x;
+  const int h = blockIdx.y;  // histogram
   if (j >= m) return;
+  part += (int64_t)h * G * m;
   double s = 0.0;
   for (int g = 0; g < G; ++g) s += part[(int64_t)g * m + j];
-  v[j] = b[j] / s;
+  v[(int64_t)h * m + j] = b[(int64_t)h * m + j] / s;
 }
 
 // u_i = a_i / sum_j K_ij v_j, one block per row
@@ -88,34 +95,37 @@ __global__ __launch_bounds__(256) void stab_row(const TM* K, int n, int m, const
                                                 const double* a, double* u, const StabCtrl* sc) {
   if (sc->c.done) return;
   const int i = blockIdx.x;
+  const int h = blockIdx.y;  // histogram (a is shared)
+  v += (int64_t)h * m;
   double acc = 0.0;
   for (int j = threadIdx.x; j < m; j += 256) acc = fma((double)K[(int64_t)i * m + j], v[j], acc);
   __shared__ double red[4];
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) u[i] = a[i] / (red[0] + red[1] + red[2] + red[3]);
+  if (threadIdx.x == 0) u[(int64_t)h * n + i] = a[i] / (red[0] + red[1] + red[2] + red[3]);
 }
 
-__global__ void stab_copy(int n, const double* x, double* y, const StabCtrl* sc) {
+__global__ void stab_copy(int64_t n, const double* x, double* y, const StabCtrl* sc) {
   if (sc && sc->c.done) return;
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) y[i] = x[i];
 }
 
 // max|u|, max|v| and NaN over u, v; the last block sets the absorption flag
 // (a NaN makes the reference's maximum NaN and its comparison false).
-__global__ __launch_bounds__(256) void stab_check(const double* u, int n, const double* v, int m,
-                                                  double tau, double* blk, StabCtrl* sc) {
+__global__ __launch_bounds__(256) void stab_check(const double* u, int64_t n, const double* v,
+                                                  int64_t m, double tau, double* blk,
+                                                  StabCtrl* sc) {
   if (sc->c.done) return;
   double mu = 0.0, mv = 0.0;
   int nan = 0;
-  const int stride = gridDim.x * 256;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
     const double x = fabs(u[i]);
     if (x != x) nan = 1; else mu = fmax(mu, x);
   }
-  for (int j = blockIdx.x * 256 + threadIdx.x; j < m; j += stride) {
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < m; j += stride) {
     const double x = fabs(v[j]);
     if (x != x) nan = 1; else mv = fmax(mv, x);
   }
@@ -228,10 +238,100 @@ __global__ __launch_bounds__(256) void stab_err(const double* part, int G, int m
   }
 }
 
-__global__ void stab_revert(double* u, const double* up, int n, double* v, const double* vp,
-                            int m, StabCtrl* sc) {
-  if (sc->c.done != 2) return;
+// Several histograms (:1275-1283 with n_hists): alpha_i += reg max_h log u_hi,
+// beta_j += reg max log v (one maximum over all of v), and u, v keep their
+// values (the reference resets them only for a single histogram).
+__global__ void stab_absorb_multi(double* alpha, const double* u, int n, int H, double* beta,
+                                  int m, double reg, const StabCtrl* sc) {
+  if (sc->c.done || !sc->absorb) return;
   const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) {
+    double mx = u[k];
+    for (int h = 1; h < H; ++h) mx = fmax(mx, u[(int64_t)h * n + k]);
+    alpha[k] += reg * log(mx);
+  }
+  if (k < m) beta[k] += reg * log(sc->maxv);  // v > 0 here, so max log v = log max |v|
+}
+
+// Several histograms: the check of :1290-1296, the relative changes
+// err = (max|u - u'| / max(max|u|, max|u'|, 1) + the same for v) / 2, then
+// the stop / NaN / max-iteration decisions of stab_err.
+__global__ __launch_bounds__(256) void stab_err_multi(const double* u, const double* up,
+                                                      int64_t nu, const double* v,
+                                                      const double* vp, int64_t nv, int check,
+                                                      int iter, int max_iter, double stop_thr,
+                                                      double* err_hist, int err_cap,
+                                                      double* blk, StabCtrl* sc) {
+  if (sc->c.done) return;
+  double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // |du|, |u|, |u'|, |dv|, |v|, |v'|
+  if (check) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nu; i += stride) {
+      r[0] = fmax(r[0], fabs(u[i] - up[i]));
+      r[1] = fmax(r[1], fabs(u[i]));
+      r[2] = fmax(r[2], fabs(up[i]));
+    }
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < nv; j += stride) {
+      r[3] = fmax(r[3], fabs(v[j] - vp[j]));
+      r[4] = fmax(r[4], fabs(v[j]));
+      r[5] = fmax(r[5], fabs(vp[j]));
+    }
+  }
+  __shared__ double red[6][4];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double x = wave_max(r[q]);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      blk[6 * blockIdx.x + q] = fmax(fmax(red[q][0], red[q][1]), fmax(red[q][2], red[q][3]));
+  }
+  __threadfence();
+  __shared__ int last;
+  if (threadIdx.x == 0) last = atomicAdd(&sc->c.counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last || threadIdx.x != 0) return;
+  __threadfence();
+  sc->c.counter = 0;
+  if (check) {
+    double R[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < (int)gridDim.x; ++k)
+      for (int q = 0; q < 6; ++q) R[q] = fmax(R[q], blk[6 * k + q]);
+    const double eu = R[0] / fmax(fmax(R[1], R[2]), 1.0);
+    const double evv = R[3] / fmax(fmax(R[4], R[5]), 1.0);
+    const double err = 0.5 * (eu + evv);
+    sc->c.err = err;
+    const int k = sc->c.n_err;
+    if (k < err_cap) err_hist[k] = err;
+    sc->c.n_err = k + 1;
+  }
+  if (sc->c.err <= stop_thr) {
+    sc->c.done = 1;
+    sc->c.status = kConverged;
+    sc->c.niter = iter;
+    return;
+  }
+  if (sc->anynan) {
+    sc->c.done = 2;
+    sc->c.status = kNumerical;
+    sc->c.niter = iter;
+    sc->c.bad_iter = iter;
+    return;
+  }
+  if (iter >= max_iter - 1) {
+    sc->c.done = 1;
+    sc->c.status = kMaxIter;
+    sc->c.niter = iter;
+  }
+}
+
+__global__ void stab_revert(double* u, const double* up, int64_t n, double* v, const double* vp,
+                            int64_t m, StabCtrl* sc) {
+  if (sc->c.done != 2) return;
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (k < n) u[k] = up[k];
   if (k < m) v[k] = vp[k];
 }
@@ -240,8 +340,9 @@ __global__ void stab_finish(StabCtrl* sc) {
   if (sc->c.done == 2) sc->c.done = 1;
 }
 
-__global__ void stab_init(StabCtrl* sc, double* u, int n, double* v, int m) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
+__global__ void stab_init(StabCtrl* sc, double* u, int64_t n, double uval, double* v, int64_t m,
+                          double vval) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (k == 0) {
     sc->c.done = 0;
     sc->c.status = 0;
@@ -259,8 +360,8 @@ __global__ void stab_init(StabCtrl* sc, double* u, int n, double* v, int m) {
     sc->maxu = 0.0;
     sc->maxv = 0.0;
   }
-  if (k < n) u[k] = 1.0 / n;
-  if (k < m) v[k] = 1.0 / m;
+  if (k < n) u[k] = uval;
+  if (k < m) v[k] = vval;
 }
 
 // Final marginal errors of Gamma = K u v: sq[0] = ||Gamma^T 1 - b||^2,
@@ -318,35 +419,39 @@ struct StabLayout {
   size_t bytes;
 };
 
-StabLayout carve_stab(void* ws, int dtype, int n, int m) {
+StabLayout carve_stab(void* ws, int dtype, int n, int m, int H) {
   Carver c(ws);
   StabLayout L;
   const size_t es = dtype == POTAMD_F32 ? 4 : 8;
   const int G = (n + kColRows - 1) / kColRows;
   L.K = c.take<char>((size_t)n * m * es);
-  L.part = c.take<double>((size_t)G * m);
-  L.up = c.take<double>(n);
-  L.vp = c.take<double>(m);
-  L.blk = c.take<double>(std::max(3 * 1024, (m + 255) / 256 + 8));
+  L.part = c.take<double>((size_t)G * m * H);
+  L.up = c.take<double>((size_t)n * H);
+  L.vp = c.take<double>((size_t)m * H);
+  L.blk = c.take<double>(std::max(6 * 1024, (m + 255) / 256 + 8));
   L.rowerr = c.take<double>(n);
   L.sc = c.take<StabCtrl>(1);
   L.bytes = c.off;
   return L;
 }
 
+// H histograms (u: H x n, v and b: H x m, histogram-major); `multi` gives
+// the reference's several-histogram branch (a 2-D b, even of one column).
 template <typename TM>
-int run_stabilized(const TM* M, int64_t ldm, int n, int m, double reg, const double* a,
-                   const double* b, double* alpha, double* beta, double* u, double* v,
-                   double tau, int max_iter, double stop_thr, int print_period, double* err_hist,
-                   int err_cap, double* marg_sq, const StabLayout& L, void* host_ctrl,
-                   hipStream_t s, int* info) {
+int run_stabilized(const TM* M, int64_t ldm, int n, int m, int H, bool multi, double reg,
+                   const double* a, const double* b, double* alpha, double* beta, double* u,
+                   double* v, double tau, int max_iter, double stop_thr, int print_period,
+                   double* err_hist, int err_cap, double* marg_sq, const StabLayout& L,
+                   void* host_ctrl, hipStream_t s, int* info) {
   TM* K = static_cast<TM*>(L.K);
   const int G = (n + kColRows - 1) / kColRows;
-  const int nm = std::max(n, m);
-  const dim3 g1((nm + 255) / 256);
+  const int64_t nu = (int64_t)n * H, nv = (int64_t)m * H;
+  const int64_t nm = std::max(nu, nv);
+  const dim3 g1((unsigned)((nm + 255) / 256));
+  const dim3 g1s((unsigned)((std::max(n, m) + 255) / 256));
   const int kblocks = (int)std::min<int64_t>(4096, ((int64_t)n * m + 255) / 256);
-  const int cblocks = std::min(1024, (nm + 255) / 256);
-  hipLaunchKernelGGL(stab_init, g1, dim3(256), 0, s, L.sc, u, n, v, m);
+  const int cblocks = (int)std::min<int64_t>(1024, (nm + 255) / 256);
+  hipLaunchKernelGGL(stab_init, g1, dim3(256), 0, s, L.sc, u, nu, 1.0 / n, v, nv, 1.0 / m);
   hipLaunchKernelGGL((stab_build_k<TM>), dim3(kblocks), dim3(256), 0, s, M, ldm, n, m, reg,
                      alpha, beta, K, (const StabCtrl*)nullptr);
   HIP_TRY(hipGetLastError());
@@ -361,25 +466,36 @@ int run_stabilized(const TM* M, int64_t ldm, int n, int m, double reg, const dou
   int rc = 0;
   StabCtrl local;
   for (int ii = 0; ii < max_iter && !stopped; ++ii) {
-    hipLaunchKernelGGL(stab_copy, dim3((n + 255) / 256), dim3(256), 0, s, n, u, L.up, L.sc);
-    hipLaunchKernelGGL(stab_copy, dim3((m + 255) / 256), dim3(256), 0, s, m, v, L.vp, L.sc);
-    hipLaunchKernelGGL((stab_col_partial<TM>), dim3((m + 255) / 256, G), dim3(256), 0, s, K, n,
-                       m, u, L.part, L.sc);
-    hipLaunchKernelGGL(stab_col_update, dim3((m + 255) / 256), dim3(256), 0, s, L.part, G, m, b,
-                       v, L.sc);
-    hipLaunchKernelGGL((stab_row<TM>), dim3(n), dim3(256), 0, s, K, n, m, v, a, u, L.sc);
-    hipLaunchKernelGGL(stab_check, dim3(cblocks), dim3(256), 0, s, u, n, v, m, tau, L.blk,
+    hipLaunchKernelGGL(stab_copy, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, nu, u,
+                       L.up, L.sc);
+    hipLaunchKernelGGL(stab_copy, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, nv, v,
+                       L.vp, L.sc);
+    hipLaunchKernelGGL((stab_col_partial<TM>), dim3((m + 255) / 256, G, H), dim3(256), 0, s, K,
+                       n, m, u, L.part, L.sc);
+    hipLaunchKernelGGL(stab_col_update, dim3((m + 255) / 256, H), dim3(256), 0, s, L.part, G, m,
+                       b, v, L.sc);
+    hipLaunchKernelGGL((stab_row<TM>), dim3(n, H), dim3(256), 0, s, K, n, m, v, a, u, L.sc);
+    hipLaunchKernelGGL(stab_check, dim3(cblocks), dim3(256), 0, s, u, nu, v, nv, tau, L.blk,
                        L.sc);
-    hipLaunchKernelGGL(stab_absorb, g1, dim3(256), 0, s, alpha, u, n, beta, v, m, reg, L.sc);
+    if (multi)
+      hipLaunchKernelGGL(stab_absorb_multi, g1s, dim3(256), 0, s, alpha, u, n, H, beta, m, reg,
+                         L.sc);
+    else
+      hipLaunchKernelGGL(stab_absorb, g1s, dim3(256), 0, s, alpha, u, n, beta, v, m, reg, L.sc);
     hipLaunchKernelGGL((stab_build_k<TM>), dim3(kblocks), dim3(256), 0, s, M, ldm, n, m, reg,
                        alpha, beta, K, L.sc);
     const int check = ii % print_period == 0;
-    if (check)
-      hipLaunchKernelGGL((stab_col_partial<TM>), dim3((m + 255) / 256, G), dim3(256), 0, s, K,
-                         n, m, u, L.part, L.sc);
-    hipLaunchKernelGGL(stab_err, dim3((m + 255) / 256), dim3(256), 0, s, L.part, G, m, b, v,
-                       check, ii, max_iter, stop_thr, err_hist, err_cap, L.blk, L.sc);
-    hipLaunchKernelGGL(stab_revert, g1, dim3(256), 0, s, u, L.up, n, v, L.vp, m, L.sc);
+    if (multi) {
+      hipLaunchKernelGGL(stab_err_multi, dim3(cblocks), dim3(256), 0, s, u, L.up, nu, v, L.vp,
+                         nv, check, ii, max_iter, stop_thr, err_hist, err_cap, L.blk, L.sc);
+    } else {
+      if (check)
+        hipLaunchKernelGGL((stab_col_partial<TM>), dim3((m + 255) / 256, G), dim3(256), 0, s, K,
+                           n, m, u, L.part, L.sc);
+      hipLaunchKernelGGL(stab_err, dim3((m + 255) / 256), dim3(256), 0, s, L.part, G, m, b, v,
+                         check, ii, max_iter, stop_thr, err_hist, err_cap, L.blk, L.sc);
+    }
+    hipLaunchKernelGGL(stab_revert, g1, dim3(256), 0, s, u, L.up, nu, v, L.vp, nv, L.sc);
     hipLaunchKernelGGL(stab_finish, dim3(1), dim3(1), 0, s, L.sc);
     if (hipGetLastError() != hipSuccess) {
       rc = POTAMD_EHIP;
@@ -415,7 +531,7 @@ int run_stabilized(const TM* M, int64_t ldm, int n, int m, double reg, const dou
   }
   for (int k = 0; k < kRing; ++k) (void)hipEventDestroy(ev[k]);
   if (rc) return rc;
-  if (marg_sq) {
+  if (marg_sq && !multi) {
     // the stop flag is set by now; rebuild the partials without it
     hipLaunchKernelGGL((stab_col_partial<TM>), dim3((m + 255) / 256, G), dim3(256), 0, s, K, n,
                        m, u, L.part, (const StabCtrl*)nullptr);
@@ -446,7 +562,36 @@ using namespace potamd;
 
 extern "C" size_t potamd_stabilized_workspace_size(int dtype, int n, int m) {
   if ((dtype != POTAMD_F32 && dtype != POTAMD_F64) || n < 1 || m < 1) return 0;
-  return carve_stab(nullptr, dtype, n, m).bytes;
+  return carve_stab(nullptr, dtype, n, m, 1).bytes;
+}
+
+extern "C" size_t potamd_stabilized_multi_workspace_size(int dtype, int n, int m, int H) {
+  if ((dtype != POTAMD_F32 && dtype != POTAMD_F64) || n < 1 || m < 1 || H < 1) return 0;
+  return carve_stab(nullptr, dtype, n, m, H).bytes;
+}
+
+extern "C" int potamd_sinkhorn_stabilized_multi(int dtype, const void* M, int64_t ldm, int n,
+                                                int m, int H, double reg, const double* a,
+                                                const double* b, double* alpha, double* beta,
+                                                double* u, double* v, double tau, int max_iter,
+                                                double stop_thr, int print_period,
+                                                double* err_hist, int err_cap, void* workspace,
+                                                size_t ws_bytes, void* host_ctrl, void* stream,
+                                                int* info) {
+  if ((dtype != POTAMD_F32 && dtype != POTAMD_F64) || !M || !a || !b || !alpha || !beta || !u ||
+      !v || !info || n < 1 || m < 1 || H < 1 || H > 65535 || ldm < m || !(reg > 0.0) ||
+      max_iter < 1 || print_period < 1 || (err_cap > 0 && !err_hist))
+    return POTAMD_EINVAL;
+  StabLayout L = carve_stab(workspace, dtype, n, m, H);
+  if (!workspace || ws_bytes < L.bytes) return POTAMD_EWORKSPACE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == POTAMD_F32)
+    return run_stabilized<float>(static_cast<const float*>(M), ldm, n, m, H, true, reg, a, b,
+                                 alpha, beta, u, v, tau, max_iter, stop_thr, print_period,
+                                 err_hist, err_cap, nullptr, L, host_ctrl, s, info);
+  return run_stabilized<double>(static_cast<const double*>(M), ldm, n, m, H, true, reg, a, b,
+                                alpha, beta, u, v, tau, max_iter, stop_thr, print_period,
+                                err_hist, err_cap, nullptr, L, host_ctrl, s, info);
 }
 
 extern "C" int potamd_sinkhorn_stabilized(int dtype, const void* M, int64_t ldm, int n, int m,
@@ -460,14 +605,14 @@ extern "C" int potamd_sinkhorn_stabilized(int dtype, const void* M, int64_t ldm,
       !v || !info || n < 1 || m < 1 || ldm < m || !(reg > 0.0) || max_iter < 1 ||
       print_period < 1 || (err_cap > 0 && !err_hist))
     return POTAMD_EINVAL;
-  StabLayout L = carve_stab(workspace, dtype, n, m);
+  StabLayout L = carve_stab(workspace, dtype, n, m, 1);
   if (!workspace || ws_bytes < L.bytes) return POTAMD_EWORKSPACE;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dtype == POTAMD_F32)
-    return run_stabilized<float>(static_cast<const float*>(M), ldm, n, m, reg, a, b, alpha, beta,
-                                 u, v, tau, max_iter, stop_thr, print_period, err_hist, err_cap,
-                                 marg_sq, L, host_ctrl, s, info);
-  return run_stabilized<double>(static_cast<const double*>(M), ldm, n, m, reg, a, b, alpha, beta,
-                                u, v, tau, max_iter, stop_thr, print_period, err_hist, err_cap,
-                                marg_sq, L, host_ctrl, s, info);
+    return run_stabilized<float>(static_cast<const float*>(M), ldm, n, m, 1, false, reg, a, b,
+                                 alpha, beta, u, v, tau, max_iter, stop_thr, print_period,
+                                 err_hist, err_cap, marg_sq, L, host_ctrl, s, info);
+  return run_stabilized<double>(static_cast<const double*>(M), ldm, n, m, 1, false, reg, a, b,
+                                alpha, beta, u, v, tau, max_iter, stop_thr, print_period,
+                                err_hist, err_cap, marg_sq, L, host_ctrl, s, info);
 }

@@ -9,9 +9,10 @@ the source marginal is checked when ``(i_ot + 1) % 10 == 0`` with a non-strict
 ``niter``.  float64 points run the fp64 pass on float64 coordinates,
 float32 points the fp32 MFMA pass.  The lazy path supports
 ``metric="sqeuclidean"``; ``batchSize`` is validated as in the reference but
-the GPU pass tiles the cost itself.  The
-reference's ``log["lazy_plan"]`` (a LazyTensor) is not produced; the plan of
-given potentials comes from ``pot_amd.online.online_plan``.
+the GPU pass tiles the cost itself.  With ``log`` the lazy path also returns
+``log["lazy_plan"]``, the plan as a ``LazyTensor`` whose blocks
+``exp(f_i + g_j - C_ij / reg)`` are computed on the GPU from the points when
+indexed (``get_sinkhorn_lazytensor``, _empirical.py:26-71).
 """
 
 import numpy as np
@@ -20,7 +21,65 @@ import torch
 from . import online
 from ._device import Ingress
 from .bregman import sinkhorn, sinkhorn2
-from .utils import dist, list_to_array
+from .utils import LazyTensor, dist, list_to_array
+
+
+def _index(k, size, dev):
+    """A LazyTensor index as a device index tensor (ints count as a slice of one)."""
+    if isinstance(k, (int, np.integer)):
+        k = int(k)
+        if not -size <= k < size:
+            raise IndexError(f"index {k} is out of bounds for size {size}")
+        k = slice(k % size, k % size + 1)
+    if isinstance(k, slice):
+        return torch.arange(size, device=dev)[k]
+    idx = torch.as_tensor(np.asarray(k) if not isinstance(k, torch.Tensor) else k)
+    if idx.dtype == torch.bool:
+        idx = torch.nonzero(idx).reshape(-1)
+    idx = idx.to(device=dev, dtype=torch.int64).reshape(-1)
+    if idx.numel() and (int(idx.max()) >= size or int(idx.min()) < -size):
+        raise IndexError(f"index out of bounds for size {size}")
+    return idx % size if idx.numel() else idx
+
+
+def get_sinkhorn_lazytensor(X_a, X_b, f, g, metric="sqeuclidean", reg=1e-1, nx=None):
+    """The Sinkhorn plan ``T = exp(f 1^T + 1 g^T - C / reg)`` of the dual
+    potentials as a LazyTensor (ot/bregman/_empirical.py:26-71): indexing it
+    evaluates the requested block on the GPU (the squared Euclidean cost
+    recomputed from the points in fp64 arithmetic, ``potamd_online_plan``)
+    and returns it in the kind and dtype of the inputs; integers select one
+    row or column as a slice of one, as in the reference."""
+    if metric != "sqeuclidean":
+        raise NotImplementedError("the GPU lazy plan computes the 'sqeuclidean' cost")
+    shape = (X_a.shape[0], X_b.shape[0])
+    ing = Ingress(X_a, X_b, f, g)
+    f64 = ing.dtype == torch.float64
+    pdt = torch.float64 if f64 else torch.float32
+    dev = {}
+
+    def resident():
+        # device copies made on first use and kept with the tensor
+        if not dev:
+            dev["Xa"] = ing.dev(X_a, pdt)
+            dev["Xb"] = ing.dev(X_b, pdt)
+            dev["f"] = ing.dev(f, torch.float64).reshape(-1)
+            dev["g"] = ing.dev(g, torch.float64).reshape(-1)
+        return dev
+
+    def func(i, j, X_a, X_b, f, g, metric, reg):
+        d = resident()
+        ii = _index(i, shape[0], ing.device)
+        jj = _index(j, shape[1], ing.device)
+        if ii.numel() == 0 or jj.numel() == 0:
+            return ing.out(torch.zeros((ii.numel(), jj.numel()), dtype=torch.float64,
+                                       device=ing.device))
+        P, _ = online.online_plan(d["Xa"].index_select(0, ii).contiguous(),
+                                  d["Xb"].index_select(0, jj).contiguous(), reg,
+                                  d["f"].index_select(0, ii), d["g"].index_select(0, jj), 1.0,
+                                  want_plan=True, device=ing.device, fp64_points=f64)
+        return ing.out(P)
+
+    return LazyTensor(shape, func, X_a=X_a, X_b=X_b, f=f, g=g, metric=metric, reg=reg)
 
 
 def _lazy(X_s, X_t, reg, a, b, metric, numIterMax, stopThr, batchSize, verbose, warn,
@@ -59,7 +118,9 @@ def empirical_sinkhorn(X_s, X_t, reg, a=None, b=None, metric="sqeuclidean", numI
         f_o, g_o = ing.out(f), ing.out(g)
         if log:
             return f_o, g_o, {"err": [ing.scalar(e) for e in lg["err"]], "u": f_o, "v": g_o,
-                              "niter": lg["niter"]}
+                              "niter": lg["niter"],
+                              "lazy_plan": get_sinkhorn_lazytensor(X_s, X_t, f_o, g_o, metric,
+                                                                   reg)}
         return f_o, g_o
     M = dist(X_s, X_t, metric=metric)
     if a is None:
@@ -88,7 +149,8 @@ def empirical_sinkhorn2(X_s, X_t, reg, a=None, b=None, metric="sqeuclidean",
         if log:
             f_o, g_o = ing.out(f), ing.out(g)
             return loss, {"err": [ing.scalar(e) for e in lg["err"]], "u": f_o, "v": g_o,
-                          "niter": lg["niter"]}
+                          "niter": lg["niter"],
+                          "lazy_plan": get_sinkhorn_lazytensor(X_s, X_t, f_o, g_o, metric, reg)}
         return loss
     M = dist(X_s, X_t, metric=metric)
     if a is None:

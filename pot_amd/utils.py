@@ -7,6 +7,7 @@
                            zero diagonal when ``X is Y``)
 * ``dist``                 ot/utils.py:400-527 ('sqeuclidean' / 'euclidean' on the GPU)
 * ``OTResult``             ot/utils.py:1127-1400 (the fields the batch solvers fill)
+* ``LazyTensor``           ot/utils.py:1425-1492 (a tensor computed from slices on demand)
 """
 
 import numpy as np
@@ -139,3 +140,36 @@ class OTResult:
     @property
     def citation(self):
         return ""
+
+
+class LazyTensor:
+    """A tensor that is never stored: ``T[key]`` calls ``getitem(i, j, ...,
+    **kwargs)`` on the indices (ot.utils.LazyTensor, ot/utils.py:1425-1492).
+
+    An integer or a slice indexes the first axis (the others are taken
+    whole), a tuple indexes the leading axes (the rest taken whole); any
+    other key raises NotImplementedError.  The keyword arguments are kept as
+    attributes of the same names.
+    """
+
+    def __init__(self, shape, getitem, **kwargs):
+        self._getitem = getitem
+        self.shape = tuple(shape)
+        self.ndim = len(self.shape)
+        self.kwargs = kwargs
+        for key, value in kwargs.items():
+            setattr(self, key, value)
+
+    def __getitem__(self, key):
+        if isinstance(key, (int, np.integer, slice)):
+            k = [key]
+        elif isinstance(key, tuple):
+            k = list(key)
+        else:
+            raise NotImplementedError("Only integer, slice, and tuple indexing is supported")
+        k += [slice(None)] * (self.ndim - len(k))
+        return self._getitem(*k, **self.kwargs)
+
+    def __repr__(self):
+        return "LazyTensor(shape={},attributes=({}))".format(self.shape,
+                                                              ",".join(self.kwargs.keys()))

@@ -110,7 +110,14 @@ def test_dense_and_online_16384_against_fp64():
                                          precision=prec, cost_scale=cs, warn=False)
         results[f"online {prec}"] = (u, v)
     torch.cuda.synchronize()
-    bars = {"online f32": (1e-5, 1e-6), "online bf16x3": (2e-5, 1e-6)}
+    # The fp32 evaluation of the cost (|x|^2 + |y|^2 - 2 x.y of fp32 points)
+    # rounds it as storing it in fp32 does, and that rounding alone moves the
+    # log-plan by lp_in (2.9e-5 at this reg and scale: the fp64 solve of the
+    # fp32-rounded cost against the fp64 cost).  The fp32 online pass is held
+    # to the larger of 1e-5 and 1.2 lp_in, the compensated bf16 pass to the
+    # larger of 2e-5 and 1.2 lp_in; their costs to 1e-6.
+    bars = {"online f32": (max(1e-5, 1.2 * lp_in), 1e-6),
+            "online bf16x3": (max(2e-5, 1.2 * lp_in), 1e-6)}
     for name, (u, v) in results.items():
         _, s = online.online_plan(X, Y, reg, u, v, cs, at, bt)
         lp = _logplan_err(u, v, u64, v64)

@@ -64,3 +64,44 @@ def test_empirical_non_lazy_and_errors():
         pot_amd.empirical_sinkhorn(xs, xt, 1.0, isLazy=True, batchSize=(1, 2, 3))
     with pytest.raises(NotImplementedError):
         pot_amd.empirical_sinkhorn(xs, xt, 1.0, isLazy=True, metric="cityblock")
+
+
+def test_lazy_plan_blocks_match_the_dense_plan():
+    # test_bregman.py:1197-1277: log["lazy_plan"][:] equals exp(f + g - M / reg)
+    n = 10
+    X_s = np.reshape(np.arange(n, dtype=np.float64), (n, 1))
+    X_t = np.reshape(np.arange(0, n, dtype=np.float64), (n, 1))
+    M = (X_s - X_t.T) ** 2
+    f, g, lg = pot_amd.empirical_sinkhorn(X_s, X_t, 1, numIterMax=1000, isLazy=True,
+                                          batchSize=5, log=True)
+    G_log = np.exp(f[:, None] + g[None, :] - M)
+    T = lg["lazy_plan"]
+    assert T.shape == (n, n)
+    assert repr(T) == "LazyTensor(shape=(10, 10),attributes=(X_a,X_b,f,g,metric,reg))"
+    np.testing.assert_allclose(T[:], G_log, rtol=1e-12, atol=1e-300)
+    loss, lg2 = pot_amd.empirical_sinkhorn2(X_s, X_t, 1, numIterMax=1000, isLazy=True,
+                                            batchSize=5, log=True)
+    np.testing.assert_allclose(lg2["lazy_plan"][:], G_log, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(loss, np.sum(G_log * M), rtol=1e-10)
+    # an integer selects a row or column as a slice of one; slices and strides
+    assert T[1].shape == (1, n) and T[1, 1].shape == (1, 1)
+    np.testing.assert_allclose(T[1, 1][0, 0], G_log[1, 1], rtol=1e-12)
+    np.testing.assert_allclose(T[::3, 5], G_log[::3, 5:6], rtol=1e-12)
+    np.testing.assert_allclose(T[2:7, 1:9], G_log[2:7, 1:9], rtol=1e-12)
+    with pytest.raises(NotImplementedError):
+        T["error"]
+
+
+def test_lazy_plan_torch_f32_points():
+    xs, xt = _samples(300, 260, 8, seed=4)
+    X = torch.tensor(xs.astype(np.float32), device="cuda")
+    Y = torch.tensor(xt.astype(np.float32), device="cuda")
+    f, g, lg = pot_amd.empirical_sinkhorn(X, Y, 2.0, isLazy=True, log=True, stopThr=1e-7)
+    P = lg["lazy_plan"][:]
+    assert isinstance(P, torch.Tensor) and P.dtype == torch.float32 and P.shape == (300, 260)
+    x64, y64 = X.double().cpu().numpy(), Y.double().cpu().numpy()
+    M = ((x64[:, None, :] - y64[None]) ** 2).sum(-1)
+    Pr = np.exp(f.double().cpu().numpy()[:, None] + g.double().cpu().numpy()[None] - M / 2.0)
+    np.testing.assert_allclose(P.cpu().numpy(), Pr, rtol=2e-6, atol=1e-30)
+    rows = lg["lazy_plan"][40:45]
+    np.testing.assert_allclose(rows.cpu().numpy(), Pr[40:45], rtol=2e-6, atol=1e-30)

@@ -215,3 +215,52 @@ def test_reference_cpu_case_fixed_iterations_match_oracle():
     assert lg["niter"] == lo["niter"]
     assert _rel_plan_err(G, Go) < 1e-10
     np.testing.assert_allclose(lg["err"], lo["err"], rtol=1e-9)
+
+
+def test_sinkhorn_log_multi_histogram_one_launch():
+    # _sinkhorn.py:816-854: each column is its own sinkhorn_log solve; here
+    # they run as one batch sharing M, each stopping on its own check
+    rng = np.random.default_rng(7)
+    n, m, nh = 70, 55, 4
+    x = rng.standard_normal((n, 2))
+    y = rng.standard_normal((m, 2)) + 0.7
+    M = ((x[:, None] - y[None]) ** 2).sum(-1)
+    M /= M.max()
+    a = rng.dirichlet(np.ones(n))
+    b = np.stack([rng.dirichlet(np.full(m, c)) for c in (0.3, 1.0, 3.0, 30.0)], 1)
+    REG = 0.01  # the columns stop at 130, 110, 110 and 100 iterations
+    loss, lg = pot_amd.sinkhorn(a, b, M, REG, method="sinkhorn_log", stopThr=1e-9,
+                                log=True)
+    lr, lgr = osk.sinkhorn_log(a, b, M, REG, stopThr=1e-9, log=True)
+    iters = [osk.sinkhorn_log(a, b[:, k], M, REG, stopThr=1e-9, log=True)[1]["niter"]
+             for k in range(nh)]
+    assert len(set(iters)) > 1, iters  # the columns stop at different iterations
+    np.testing.assert_allclose(loss, lr, rtol=1e-10)
+    for key in ("log_u", "log_v"):
+        assert lg[key].shape == lgr[key].shape
+        np.testing.assert_allclose(lg[key], lgr[key], atol=1e-9)
+    np.testing.assert_allclose(lg["u"], lgr["u"], rtol=1e-8)
+    # sinkhorn2 and the torch path return the same losses
+    l2 = pot_amd.sinkhorn2(a, b, M, REG, method="sinkhorn_log", stopThr=1e-9)
+    np.testing.assert_allclose(l2, lr, rtol=1e-10)
+    lt = pot_amd.sinkhorn2(torch.tensor(a, device="cuda"), torch.tensor(b, device="cuda"),
+                           torch.tensor(M, device="cuda"), REG, method="sinkhorn_log",
+                           stopThr=1e-9)
+    assert isinstance(lt, torch.Tensor) and lt.shape == (nh,)
+    np.testing.assert_allclose(lt.cpu().numpy(), lr, rtol=1e-10)
+    # warm starts per column
+    ws = [(lgr["log_u"][:, k], lgr["log_v"][:, k]) for k in range(nh)]
+    lw = pot_amd.sinkhorn2(a, b, M, REG, method="sinkhorn_log", stopThr=1e-9, warmstart=ws)
+    lwr = osk.sinkhorn2(a, b, M, REG, method="sinkhorn_log", stopThr=1e-9, warmstart=ws)
+    np.testing.assert_allclose(lw, lwr, rtol=1e-10)
+    # one warning per column that does not converge (the per-column calls warn)
+    with warnings.catch_warnings(record=True) as rec:
+        warnings.simplefilter("always")
+        pot_amd.sinkhorn(a, b, M, REG, method="sinkhorn_log", stopThr=1e-9, numItermax=105)
+    with warnings.catch_warnings(record=True) as rec_r:
+        warnings.simplefilter("always")
+        osk.sinkhorn_log(a, b, M, REG, stopThr=1e-9, numItermax=105)
+    assert len(rec) == len(rec_r) == 3  # only the column stopping at 100 converges
+    # b of shape (m, 1) without a warm start raises, as in the reference
+    with pytest.raises(TypeError):
+        pot_amd.sinkhorn(a, b[:, :1], M, REG, method="sinkhorn_log")

@@ -77,3 +77,32 @@ def test_stabilized_f32_and_eps_scaling():
     assert lg["niter"] == lr["niter"]
     np.testing.assert_allclose(lg["err"], lr["err"], rtol=1e-6, atol=1e-15)
     assert _rel(G, Gr) < 1e-9
+
+
+@pytest.mark.parametrize("tau", [1e3, 30.0])
+def test_stabilized_multi_histogram_matches_oracle(tau):
+    # the several-histogram branch (_sinkhorn.py:1230-1367) on the device:
+    # same n_iter, err list, losses and logs as the float64 restatement
+    from oracle.stabilized import sinkhorn_stabilized_multi
+    rng = np.random.default_rng(11)
+    n, m, H = 60, 45, 3
+    M = rng.random((n, m))
+    a = rng.dirichlet(np.ones(n))
+    B = np.stack([rng.dirichlet(np.ones(m)) for _ in range(H)], 1)
+    loss, lg = pot_amd.sinkhorn(a, B, M, 0.02, method="sinkhorn_stabilized", tau=tau,
+                                stopThr=1e-9, numItermax=3000, log=True)
+    lr, lgr = sinkhorn_stabilized_multi(a, B, M, 0.02, tau=tau, stopThr=1e-9, numItermax=3000,
+                                        log=True)
+    assert lg["n_iter"] == lgr["n_iter"]
+    np.testing.assert_allclose(lg["err"], lgr["err"], rtol=1e-8, atol=1e-15)
+    np.testing.assert_allclose(loss, lr, rtol=1e-10)
+    for key in ("logu", "logv", "alpha", "beta"):
+        assert lg[key].shape == lgr[key].shape, key
+        np.testing.assert_allclose(lg[key], lgr[key], rtol=1e-9, atol=1e-9)
+    if tau == 30.0:  # absorbed: the error path differs from the run without absorption
+        _, l_free = sinkhorn_stabilized_multi(a, B, M, 0.02, tau=1e3, stopThr=1e-9,
+                                              numItermax=3000, log=True)
+        assert l_free["err"][1] != lgr["err"][1]
+    l2 = pot_amd.sinkhorn2(a, B, M, 0.02, method="sinkhorn_stabilized", tau=tau, stopThr=1e-9,
+                           numItermax=3000)
+    np.testing.assert_allclose(l2, lr, rtol=1e-10)

@@ -167,3 +167,26 @@ def test_points_and_network_argument_validation():
     assert mlp(seg=100, rows=1000) == -3     # segments must be whole 64-row blocks
     assert mlp(inp=None) == -1               # no input rows and no points
     assert mlp(inp=Q) == -1                  # 16-byte loads need alignment
+
+
+def test_lazy_tensor_indexing_semantics():
+    # test/test_utils.py:522-558 with a NumPy getitem
+    import numpy as np
+
+    from pot_amd.utils import LazyTensor
+    rng = np.random.RandomState(42)
+    x1, x2 = rng.randn(100, 2), rng.randn(200, 2)
+
+    def getitem(i, j, x1, x2):
+        return np.dot(x1[i], x2[j].T)
+
+    T = LazyTensor((100, 200), getitem, x1=x1, x2=x2)
+    assert T.shape == (100, 200)
+    assert str(T) == "LazyTensor(shape=(100, 200),attributes=(x1,x2))"
+    assert T.x1 is x1 and T.x2 is x2
+    assert T[:].shape == (100, 200)
+    assert T[1, 1] == np.dot(x1[1], x2[1].T)
+    assert T[1].shape == (200,)
+    assert T[::10, 5].shape == (10,)
+    with pytest.raises(NotImplementedError):
+        T["error"]

@@ -478,3 +478,33 @@ def test_stabilized_and_eps_scaling_oracle():
     np.testing.assert_allclose(P2, P_log, atol=1e-10)
     assert l1["n_iter"] % 20 == 0 and l1["err"][-1] <= 1e-12
     assert np.abs(l2["alpha"]).max() > 0  # absorbed at least once
+
+
+def test_stabilized_multi_histogram_oracle_fixed_point():
+    # the several-histogram branch (_sinkhorn.py:1230-1367) at a tight
+    # tolerance: each histogram's loss is the single-histogram loss of its
+    # column, with and without absorption; identical columns give identical
+    # scalings; a 2x2 doctest problem in two equal columns gives the doctest loss
+    from oracle.stabilized import sinkhorn_stabilized_multi
+    rng = np.random.default_rng(9)
+    n, m, H = 30, 25, 3
+    M = rng.random((n, m))
+    a = rng.dirichlet(np.ones(n))
+    B = np.stack([rng.dirichlet(np.ones(m)) for _ in range(H)], 1)
+    want = np.array([np.sum(osk.sinkhorn_log(a, B[:, h], M, 0.05, stopThr=1e-13,
+                                             numItermax=20000) * M) for h in range(H)])
+    errs = {}
+    for tau in (1e3, 30.0):
+        loss, lg = sinkhorn_stabilized_multi(a, B, M, 0.05, tau=tau, stopThr=1e-14,
+                                             numItermax=20000, log=True)
+        np.testing.assert_allclose(loss, want, rtol=1e-9)
+        assert lg["logu"].shape == (n, H) and lg["alpha"].shape == (n, H)
+        errs[tau] = lg["err"]
+    # tau = 30 absorbs (max |u| reaches the hundreds): another path, same end
+    assert errs[30.0][1] != errs[1e3][1]
+    Bsame = np.stack([B[:, 0], B[:, 0]], 1)
+    _, lg = sinkhorn_stabilized_multi(a, Bsame, M, 0.05, stopThr=1e-12, log=True)
+    np.testing.assert_array_equal(lg["logu"][:, 0], lg["logu"][:, 1])
+    l2 = sinkhorn_stabilized_multi([.5, .5], np.array([[.5, .5], [.5, .5]]),
+                                   [[0., 1.], [1., 0.]], 1, stopThr=1e-14)
+    np.testing.assert_allclose(l2, [0.26894142] * 2, atol=1e-8)  # 1 / (1 + e)
