@@ -261,6 +261,30 @@ int potamd_sinkhorn_log_batch_points_split(int batch, int max_n, int max_m, int 
                                            double* err_first, double* cmax, int* queue,
                                            void* stream);
 
+/* No-allocation variant of potamd_sinkhorn_log_batch_points_split.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The work-queue counters and the exchange slots
+ * live in a fixed device array of the library (potamd_noalloc_scratch_bytes
+ * bytes per GPU, held until the process ends); the call clears the parts it
+ * uses with a fill kernel on `stream`, then solves, and allocates nothing.
+ * nsplit: 0 .. POTAMD_NOALLOC_MAX_SPLIT (more returns POTAMD_EWORKSPACE);
+ *    with 0 every split entry must be 0 (a problem with a slot reports
+ *    status 2, as a malformed entry does).  Other arguments as for
+ *    potamd_sinkhorn_log_batch_points_split; the results are the same.
+ * Calls that overlap on one GPU share the array and race: serialise them
+ * (one stream, or events).
+ */
+#define POTAMD_NOALLOC_MAX_SPLIT 256
+size_t potamd_noalloc_scratch_bytes(void);
+int potamd_sinkhorn_log_batch_points_split_noalloc(
+    int batch, int max_n, int max_m, int d, const float* X, const float* Y, const int64_t* xoff,
+    const int64_t* yoff, const int* ns, const int* ms, const double* regs, int normalize,
+    const double* a, const double* b, double* u, double* v, int max_iter, int check_every,
+    double stop_thr, double fallback_thr, int items, const int* order, const int* split,
+    int nsplit, int* niter, int* status, double* err_last, double* err_first, double* cmax,
+    void* stream);
+
 /* Per-problem maximum of |x_r - y_c|^2 (the M.max() of ot.dist(x, y) /
  * M.max()), the same device search as potamd_sinkhorn_log_batch_points with
  * normalize = 1 (MFMA pass, then the best pair of each lane evaluated in fp64

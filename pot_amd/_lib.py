@@ -66,6 +66,15 @@ SIGNATURES = {
         c_vp, c_vp, c_vp, c_vp, c_vp,            # niter, status, err_last, err_first, cmax
         c_vp, c_vp]),                            # queue, stream
     "potamd_points_split_workspace_size": (c_size, [c_int]),
+    "potamd_sinkhorn_log_batch_points_split_noalloc": (c_int, [
+        c_int, c_int, c_int, c_int, c_vp, c_vp,  # batch, max_n, max_m, d, X, Y
+        c_vp, c_vp, c_vp, c_vp, c_vp, c_int,     # xoff, yoff, ns, ms, regs, normalize
+        c_vp, c_vp, c_vp, c_vp,                  # a, b, u, v
+        c_int, c_int, c_dbl, c_dbl,              # max_iter, check_every, stop, fallback
+        c_int, c_vp, c_vp, c_int,                # items, order, split, nsplit
+        c_vp, c_vp, c_vp, c_vp, c_vp,            # niter, status, err_last, err_first, cmax
+        c_vp]),                                  # stream
+    "potamd_noalloc_scratch_bytes": (c_size, []),
     "potamd_ragged_points_cmax": (c_int, [
         c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "potamd_sqeuclidean_ragged": (c_int, [

@@ -169,6 +169,18 @@ def test_points_and_network_argument_validation():
     assert mlp(inp=Q) == -1                  # 16-byte loads need alignment
 
 
+def test_noalloc_rejects_more_slots_than_its_scratch():
+    # checked on the host before any GPU work: nsplit above the fixed maximum
+    lib = _lib.load()
+    P = ctypes.c_void_p(0x1000)
+    assert lib.potamd_noalloc_scratch_bytes() >= 256 + 256 * 33280
+    for nsplit, want in ((257, -4), (-1, -1)):
+        rc = lib.potamd_sinkhorn_log_batch_points_split_noalloc(
+            1, 64, 64, 16, P, P, P, P, P, P, P, 1, P, P, P, P, 10, 10, 1e-6, 0.0, 1, P, P,
+            nsplit, P, P, P, P, P, ctypes.c_void_p(0))
+        assert rc == want, (nsplit, rc)
+
+
 def test_lazy_tensor_indexing_semantics():
     # test/test_utils.py:522-558 with a NumPy getitem
     import numpy as np
