@@ -73,7 +73,8 @@ def _args():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cold-only", action="store_true")
     ap.add_argument("--no-c2", action="store_true",
-                    help="warmstart: skip the 16384^2 dense figure (profiling runs)")
+                    help="warmstart: skip the 16384^2 dense and 512-problem ragged figures "
+                         "(profiling runs)")
     return ap.parse_args()
 
 
@@ -205,6 +206,84 @@ def dense_c2_figure(torch, dev, steps=2):
                          "alg_bytes_per_iteration": float(n) * n * 4,
                          "traffic": DENSE_C2_TRAFFIC,
                          "traffic_over_alg": DENSE_C2_TRAFFIC / (float(n) * n * 4)}}
+
+
+def ragged_c3_figure(torch, dev, steps=5, warmup=2, check=4, cpu=True):
+    """The uneven batch: 512 problems, n_i, m_i ~ U[64, 1024], d = 16, GMM(1-4),
+    reg_i ~ logU[5e-3, 5e-2], stopThr 1e-6 (checked every 10 iterations, the
+    reference's cadence), numItermax 2000, solved as one captured point-cloud
+    hipGraph with the automatic split; replays timed with events on the
+    solve's stream.  With ``cpu`` (the CPU-baseline leg) the float64 oracle
+    solves the ``check`` largest problems on one core, timed, and the GPU
+    solution of each is compared with it at its own stop (plan relative
+    error on entries >= 1e-30, cost relative error)."""
+    from pot_amd import problems, ragged
+    fam = problems.config3(seed=2, B=512)
+    ns = np.array([len(x) for x in fam.xs])
+    ms = np.array([len(y) for y in fam.ys])
+    Xp = torch.tensor(np.concatenate(fam.xs), device=dev)
+    Yp = torch.tensor(np.concatenate(fam.ys), device=dev)
+    G = ragged.RaggedGraph(ns, ms, 16, numItermax=2000, stopThr=1e-6, check_every=10,
+                           cost="points", split="auto")
+    for _ in range(warmup):
+        G.solve(Xp, Yp, fam.regs)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        r = G.solve(Xp, Yp, fam.regs)
+    e1.record()
+    torch.cuda.synchronize()
+    dt = e0.elapsed_time(e1) * 1e-3 / steps
+    nit = r.niter.cpu().numpy().astype(np.int64)
+    st = r.status.cpu().numpy()
+    elems = float(np.sum((nit + 1) * ns * ms))
+    tf = 64.0 * elems / dt / 1e12
+    out = {"workload": "ragged B=512 n_i,m_i~U[64,1024] d=16 GMM(1-4) reg~logU[5e-3,5e-2] "
+                       "stopThr=1e-6, one captured point-cloud hipGraph, split auto",
+           "problems_per_s": 512 / dt, "ms_per_step": 1e3 * dt,
+           "iters": {"mean": float(nit.mean()), "max": int(nit.max())},
+           "converged_frac": float(((st & 3) == 0).mean()),
+           "roofline": {"bound": "mfma", "kernel": "batch_points_kernel (split, graph replay)",
+                        "achieved": tf, "peak": MFMA_PEAK_TFLOPS["f16"], "unit": "TFLOP/s",
+                        "frac": tf / MFMA_PEAK_TFLOPS["f16"], "traffic": None,
+                        "note": "64 algorithmic flops per element-iteration over the replay"}}
+    if not cpu:
+        return out
+    from oracle.sinkhorn import sinkhorn_log
+    u_all, v_all = r.u.cpu().numpy(), r.v.cpu().numpy()
+    ao = np.concatenate([[0], np.cumsum(ns)[:-1]])
+    bo = np.concatenate([[0], np.cumsum(ms)[:-1]])
+    worst_rel, worst_cost, exact, t_cpu = 0.0, 0.0, 0, 0.0
+    for q in np.argsort(-(ns * ms))[:check]:
+        x64 = fam.xs[q].astype(np.float64)
+        y64 = fam.ys[q].astype(np.float64)
+        M = np.maximum((x64 ** 2).sum(1)[:, None] + (y64 ** 2).sum(1)[None]
+                       - 2 * x64 @ y64.T, 0.0)
+        M /= M.max()
+        a = np.full(ns[q], 1.0 / ns[q])
+        b = np.full(ms[q], 1.0 / ms[q])
+        t0 = time.perf_counter()
+        Pr, lr = sinkhorn_log(a, b, M, fam.regs[q], numItermax=2000, stopThr=1e-6, log=True,
+                              warn=False)
+        t_cpu += time.perf_counter() - t0
+        if lr["niter"] != int(nit[q]):
+            continue
+        exact += 1
+        P = np.exp(-M / fam.regs[q] + u_all[ao[q]:ao[q] + ns[q]][:, None]
+                   + v_all[bo[q]:bo[q] + ms[q]][None, :])
+        mask = Pr >= 1e-30
+        worst_rel = max(worst_rel, float(np.max(np.abs(P[mask] - Pr[mask]) / Pr[mask])))
+        worst_cost = max(worst_cost, abs(float(np.sum(P * M) - np.sum(Pr * M)))
+                         / float(np.sum(Pr * M)))
+    out["cpu_baseline"] = {"value": check / t_cpu, "unit": "problems/s", "cores": 1,
+                           "kind": "port",
+                           "sample": f"the {check} largest problems, float64 NumPy "
+                                     f"sinkhorn_log, one core, {t_cpu:.1f} s"}
+    out["vs_oracle"] = {"problems": check, "same_stop": exact, "max_plan_rel_err": worst_rel,
+                        "max_cost_rel_err": worst_cost}
+    return out
 
 
 def run_warmstart(args, torch, dev):
@@ -354,6 +433,7 @@ def run_warmstart(args, torch, dev):
         out["cpu_baseline"] = cpu_baseline_family(fam, args.stop, args.max_iter)
     if not args.no_c2:
         out["dense_c2"] = dense_c2_figure(torch, dev, steps=2)
+        out["ragged_c3"] = ragged_c3_figure(torch, dev, cpu=not args.no_cpu)
     return out
 
 

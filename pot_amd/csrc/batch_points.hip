@@ -108,42 +108,44 @@ __device__ __forceinline__ unsigned bf16_bits(float f) {
   return u >> 16;
 }
 
-// x = p[0] + p[1] + p[2] + p[3] to 2^-32 relative, each part a bf16; an
-// infinite x gives (x, 0, 0, 0).
-__device__ __forceinline__ void split4_bf16(double x, unsigned p[4]) {
-  if (!__builtin_isfinite(x)) {
-    p[0] = bf16_bits((float)x);
-    p[1] = p[2] = p[3] = 0u;
-    return;
-  }
-  double r = x;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    p[i] = bf16_bits((float)r);
-    r -= (double)__uint_as_float(p[i] << 16);
-  }
+// x = p0 + p1 + p2 + p3 to 2^-32 relative, each part a bf16, packed two per
+// word as (p0 | p1 << 16, p2 | p3 << 16); an infinite or NaN x gives
+// (x, 0, 0, 0).  Straight-line code with selects (no local array, no early
+// exit), so the parts stay in registers.
+__device__ __forceinline__ void split4_bf16(double x, unsigned& w01, unsigned& w23) {
+  const bool fin = __builtin_isfinite(x);
+  double r = fin ? x : 0.0;
+  const unsigned p0 = bf16_bits((float)r);
+  r -= (double)__uint_as_float(p0 << 16);
+  const unsigned p1 = bf16_bits((float)r);
+  r -= (double)__uint_as_float(p1 << 16);
+  const unsigned p2 = bf16_bits((float)r);
+  r -= (double)__uint_as_float(p2 << 16);
+  const unsigned p3 = bf16_bits((float)r);
+  w01 = fin ? (p0 | (p1 << 16)) : bf16_bits((float)x);
+  w23 = fin ? (p2 | (p3 << 16)) : 0u;
 }
 
 // K slots of the potential MFMA.  Column side (A operand, lanes 0-31):
 // [P1, P2, P3, P4, 1, 1, 1, 1]; row side (B operand): [1, 1, 1, 1, -Q1..-Q4].
 __device__ __forceinline__ u32x4 kslot_col(double P) {
-  unsigned p[4];
-  split4_bf16(P, p);
   u32x4 k;
-  k[0] = p[0] | (p[1] << 16);
-  k[1] = p[2] | (p[3] << 16);
+  unsigned w01, w23;
+  split4_bf16(P, w01, w23);
+  k[0] = w01;
+  k[1] = w23;
   k[2] = kOnes;
   k[3] = kOnes;
   return k;
 }
 __device__ __forceinline__ u32x4 kslot_row(double Q) {
-  unsigned q[4];
-  split4_bf16(-Q, q);
   u32x4 k;
+  unsigned w01, w23;
+  split4_bf16(-Q, w01, w23);
   k[0] = kOnes;
   k[1] = kOnes;
-  k[2] = q[0] | (q[1] << 16);
-  k[3] = q[2] | (q[3] << 16);
+  k[2] = w01;
+  k[3] = w23;
   return k;
 }
 
@@ -173,6 +175,31 @@ __device__ __forceinline__ void point_ops(const float* p, bool valid, int d, con
     hi[k] = hb[2 * k] | (hb[2 * k + 1] << 16);
     lo[k] = lb[2 * k] | (lb[2 * k + 1] << 16);
   }
+}
+
+// This lane's index in its wave, recomputed at the point of use: the asm
+// result cannot be kept live from an earlier computation, so addresses built
+// from it in the once-per-sweep sections are rebuilt there (two VALU
+// instructions) instead of being spilled across the block loop and reloaded
+// from scratch.
+__device__ __forceinline__ int lane_now() {
+  int l;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+  return l;
+}
+
+// Sum over the wave of a double, the shuffle addresses rebuilt from
+// lane_now() (the same butterfly, in the same order, as wave_sum).
+__device__ __forceinline__ double wave_sum_now(double x) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const int src = (lane_now() ^ o) << 2;
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(src, (int)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(src, (int)(b >> 32));
+    x += __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  }
+  return x;
 }
 
 // Values that every lane of the workgroup holds (read back from LDS after a
@@ -228,8 +255,10 @@ struct PointsLds {
   float sh[kPMax];                // shifted row sums (prologue: column maxima)
   float colx[kPMax];              // column sums
   // Row-sum partials [buffer][row][wave]: a row's PW partials are adjacent,
-  // so a lane reads them with 16-byte loads in one LDS round trip.
-  __attribute__((aligned(16))) float rowpart[2][32][PW];
+  // so a lane reads them with 16-byte loads in one LDS round trip.  Rows are
+  // PW + 4 floats apart (still 16-byte aligned), which spreads the 32 lanes'
+  // stores of one wave over twice as many banks as a stride of PW.
+  __attribute__((aligned(16))) float rowpart[2][32][PW + 4];
   float rowmax[PW][32];
   double cen[kPD];
   double red[PW];
@@ -339,6 +368,10 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   };
   // Tile j of this wave covers columns 32 * (wave + 8 j) + [0, 32).
   auto col_of = [&](int j) { return 32 * (wave + PW * j) + l32; };
+  // The same column, and the half of the wave, from the recomputed lane index
+  // (for the sections that run once per sweep).
+  auto col_now = [&](int j) { return 32 * (wave + PW * j) + (lane_now() & 31); };
+  auto half_now = [&]() { return lane_now() >> 5; };
   // Column of value i of tile j in the MFMA output layout.
   auto out_col = [&](int j, int i) {
     return 32 * (wave + PW * j) + 8 * (i / 4) + 4 * h + (i % 4);
@@ -536,7 +569,8 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
         const float tot = op(__uint_as_float(sw[0]), __uint_as_float(sw[1]));
         w = (l32 >> 2) == r ? tot : w;
       }
-      if (l32 < 16) dst[32 * (wave + PW * j) + 8 * (l32 >> 2) + 4 * h + (l32 & 3)] = w;
+      const int ln = lane_now(), lw = ln & 31;
+      if (lw < 16) dst[32 * (wave + PW * j) + 8 * (lw >> 2) + 4 * (ln >> 5) + (lw & 3)] = w;
     }
   };
 
@@ -791,7 +825,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
 
     // ---- row potentials, marginal error, stop decision ----
     const int rend = SPLIT ? (blk1 * 32 < n ? blk1 * 32 : n) : n;
-    for (int r = blk0 * 32 + tid; r < rend; r += PT) {
+    for (int r = blk0 * 32 + wave * kWave + lane_now(); r < rend; r += PT) {
       if (L.af[r] > 0.f) {
         // sh[r] lies in [2^-60, 2^60] (the safe range of the sweep).
         const double c = L.c2s[r] + (double)__builtin_amdgcn_logf(L.sh[r]);
@@ -802,13 +836,13 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     double e2 = 0.0;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      const int c = col_of(j);
-      if (h == 0 && c < m) {
+      const int c = col_now(j);
+      if (half_now() == 0 && c < m) {
         const double dd = (double)L.colx[c] - L.bs[c];
         e2 += dd * dd;
       }
     }
-    e2 = wave_sum(e2);
+    e2 = wave_sum_now(e2);
     if (lane == 0) L.red[wave] = e2;
     __syncthreads();
     if (tid == 0) {
@@ -846,11 +880,14 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     // ---- column update v_c += log2 b_c - log2 colsum_c (own columns) ----
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      const int c = col_of(j);
-      if (h == 0 && c < m) {
+      const int c = col_now(j);
+      if (half_now() == 0 && c < m) {
+        // log2 of the column sum on the fp32 log (the fp32 sums carry no more
+        // than that); a subnormal sum is scaled into the normal range first
         const float cs = L.colx[c];
-        const double lcs = cs >= 1.1754944e-38f ? (double)__builtin_amdgcn_logf(cs)
-                                                : log2((double)cs);
+        const bool tiny = cs < 1.1754944e-38f;
+        const double lcs = (double)__builtin_amdgcn_logf(tiny ? cs * 0x1p64f : cs) -
+                           (tiny ? 64.0 : 0.0);
         const double vn = L.lbs[c] > -__builtin_inf() ? L.vsh[c] + L.lbs[c] - lcs
                                                       : -__builtin_inf();
         L.vsh[c] = vn;
