@@ -149,21 +149,39 @@ __device__ __forceinline__ u32x4 kslot_row(double Q) {
   return k;
 }
 
-// Half a point (coordinates 8*hh .. 8*hh+7) as f16 hi and lo operands of
-// (p - cen) * sc; nrm accumulates the squared norm of the represented
-// (hi + lo) coordinates, mx the largest |coordinate|.
-__device__ __forceinline__ void point_ops(const float* p, bool valid, int d, const double* cen,
-                                          double sc, int hh, u32x4& hi, u32x4& lo, double& nrm,
-                                          float& mx) {
+// Coordinates 8*hh .. 8*hh+7 of the point at p (those past d read
+// coordinate 0 and are not used), loaded ahead of point_ops so that all the
+// loads of a pass are issued before the first is used.
+__device__ __forceinline__ void half_load(const float* p, int d, int hh, float (&v)[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int kk = 8 * hh + k;
+    v[k] = p[kk < d ? kk : 0];
+  }
+}
+// float -> f16 on v_cvt_f16_f32 (round to nearest even).  The value passes
+// through an empty asm so that the compiler cannot merge a preceding
+// double -> float conversion into one double -> f16 conversion, which it
+// would emulate in about 40 integer instructions.
+__device__ __forceinline__ _Float16 f16_of(float x) {
+  asm volatile("" : "+v"(x));
+  return (_Float16)x;
+}
+// Half a point (coordinates 8*hh .. 8*hh+7, loaded by half_load) as f16 hi
+// and lo operands of (p - cen) * sc; nrm accumulates the squared norm of the
+// represented (hi + lo) coordinates, mx the largest |coordinate|.
+__device__ __forceinline__ void point_ops(const float (&pv)[8], bool valid, int d,
+                                          const double* cen, double sc, int hh, u32x4& hi,
+                                          u32x4& lo, double& nrm, float& mx) {
   unsigned hb[8], lb[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int kk = 8 * hh + k;
     double x = 0.0;
-    if (valid && kk < d) x = ((double)p[kk] - cen[kk]) * sc;
-    const _Float16 xh = (_Float16)(float)x;
+    if (valid && kk < d) x = ((double)pv[k] - cen[kk]) * sc;
+    const _Float16 xh = f16_of((float)x);
     const double xhd = (double)(float)xh;
-    const _Float16 xl = (_Float16)(float)(x - xhd);
+    const _Float16 xl = f16_of((float)(x - xhd));
     const double xr = xhd + (double)(float)xl;
     nrm = fma(xr, xr, nrm);
     mx = fmaxf(mx, fabsf((float)x));
@@ -379,13 +397,32 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
 
   // Operand packing: rows into rop / xn, this wave's columns into registers / yn.
   u32x4 ahi[NS], alo[NS], ak[NS];
+  // Every coordinate a thread packs is loaded first (rows past n read row 0;
+  // the results are not used), so the loads of a pass overlap instead of
+  // waiting one point at a time.
+  constexpr int RQ = (kPBlocks * kWave + PT - 1) / PT;  // row half-points per thread
+  constexpr int RR = (kPMax + PT - 1) / PT;             // rows per thread
   auto pack = [&](double sc) -> float {
     float mx = 0.f;
-    for (int q = tid; q < nblk * kWave; q += PT) {
+    float rv[RQ][8], cv[NS][8];
+#pragma unroll
+    for (int u = 0; u < RQ; ++u) {
+      const int q = tid + u * PT, r = (q >> 6) * 32 + (q & 31);
+      half_load(Xp + (int64_t)(r < n ? r : 0) * d, d, (q >> 5) & 1, rv[u]);
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int c = col_of(j);
+      half_load(Yp + (int64_t)(c < m ? c : 0) * d, d, h, cv[j]);
+    }
+#pragma unroll
+    for (int u = 0; u < RQ; ++u) {
+      const int q = tid + u * PT;
+      if (q >= nblk * kWave) break;
       const int blk = q >> 6, ln = q & 63, r = blk * 32 + (ln & 31), hh = ln >> 5;
       u32x4 hi4, lo4;
       double nh = 0.0;
-      point_ops(Xp + (int64_t)(r < n ? r : 0) * d, r < n, d, L.cen, sc, hh, hi4, lo4, nh, mx);
+      point_ops(rv[u], r < n, d, L.cen, sc, hh, hi4, lo4, nh, mx);
       L.rop[blk][0][ln] = hi4;
       L.rop[blk][1][ln] = lo4;
       nh += __shfl_xor(nh, 32, kWave);
@@ -396,7 +433,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
       const int c = col_of(j);
       u32x4 hi4, lo4;
       double nh = 0.0;
-      point_ops(Yp + (int64_t)(c < m ? c : 0) * d, c < m, d, L.cen, sc, h, hi4, lo4, nh, mx);
+      point_ops(cv[j], c < m, d, L.cen, sc, h, hi4, lo4, nh, mx);
       ahi[j] = hi4;
       alo[j] = lo4;
       nh += __shfl_xor(nh, 32, kWave);
@@ -533,6 +570,21 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     const u32x4 k = L.rks[blk * 32 + l32];
     return h == 0 ? k : zero4;
   };
+  // One pass over this workgroup's row blocks: the tiles of each block into
+  // e, then body(blk); the row operands are read one block ahead (the last
+  // block reads itself again), so their LDS latency overlaps the work.
+  auto rows_pass = [&](auto&& body) {
+    u32x4 nbh = L.rop[blk0][0][lane], nbl = L.rop[blk0][1][lane], nbk = row_kslot(blk0);
+    for (int blk = blk0; blk < blk1; ++blk) {
+      const u32x4 bh = nbh, bl = nbl, bk = nbk;
+      const int nb = blk + 1 < blk1 ? blk + 1 : blk;
+      nbh = L.rop[nb][0][lane];
+      nbl = L.rop[nb][1][lane];
+      nbk = row_kslot(nb);
+      tiles_ops(bh, bl, bk);
+      body(blk);
+    }
+  };
   // Per-column reduction of acc over the rows (sum or max) into dst.
   f32x16 acc[NS];
   // Recursive halving: an exchange with lane ^ 1 pairs values i and i+1
@@ -585,35 +637,59 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     for (int r = tid; r < nblk * 32; r += PT)
       L.rks[r] = kslot_row(r < n ? L.xn[r] : -__builtin_inf());
     __syncthreads();
-    // One running minimum per column tile (independent chains), merged at
-    // the end: the first occurrence in (blk, j, i) order wins, as with one
-    // running minimum, since code grows with that order.
+    // Per column tile, each lane keeps the smallest value over the rows and
+    // the first block reaching it (the minimum over a block's 16 values is a
+    // min3 chain).  Lanes l and l ^ 32 hold the same row slot (other
+    // columns): they agree on one block per tile, the one with the smaller
+    // value (the earlier on a tie), whose tile is recomputed (the same MFMA
+    // chain, so the same values) to find the column: the first occurrence of
+    // the pair's minimum in (block, tile, value) order wins.
     float bestj[NS];
-    int codej[NS];
+    int blkj[NS];
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
       bestj[j] = __builtin_inff();
-      codej[j] = -1;
+      blkj[j] = -1;
     }
-    for (int blk = blk0; blk < blk1; ++blk) {
-      tiles(blk, row_kslot(blk));
+    rows_pass([&](int blk) {
 #pragma unroll
       for (int j = 0; j < NS; ++j) {
+        float mn = e[j][0];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const bool lt = e[j][i] < bestj[j];
-          bestj[j] = lt ? e[j][i] : bestj[j];
-          codej[j] = lt ? (blk << 8) | (j << 4) | i : codej[j];
-        }
+        for (int i = 1; i < 16; ++i) mn = fminf(mn, e[j][i]);
+        const bool lt = mn < bestj[j];
+        bestj[j] = lt ? mn : bestj[j];
+        blkj[j] = lt ? blk : blkj[j];
       }
-    }
+    });
     float best = __builtin_inff();
     int code = -1;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      const bool take = codej[j] >= 0 && (bestj[j] < best || (bestj[j] == best && codej[j] < code));
-      best = take ? bestj[j] : best;
-      code = take ? codej[j] : code;
+      const auto sv = __builtin_amdgcn_permlane32_swap(__float_as_uint(bestj[j]),
+                                                       __float_as_uint(bestj[j]), false, false);
+      const auto sb = __builtin_amdgcn_permlane32_swap((unsigned)blkj[j], (unsigned)blkj[j],
+                                                       false, false);
+      // sv / sb: [0] the value of lanes 0-31, [1] of lanes 32-63, in every lane.
+      const float v0 = __uint_as_float(sv[0]), v1 = __uint_as_float(sv[1]);
+      const int b0 = (int)sb[0], b1 = (int)sb[1];
+      const bool one = v1 < v0 || (v1 == v0 && (unsigned)b1 < (unsigned)b0);
+      const float pv = one ? v1 : v0;
+      const int pb = one ? b1 : b0;
+      const int rb = pb >= 0 ? pb : blk0;
+      const f32x16 z = {};
+      const u32x4 rh = L.rop[rb][0][lane], rl = L.rop[rb][1][lane], rk = row_kslot(rb);
+      f32x16 t = mfma_h(alo[j], rh, z);
+      t = mfma_h(ahi[j], rl, t);
+      t = mfma_h(ahi[j], rh, t);
+      t = mfma_b(ak[j], rk, t);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int cand = (pb << 8) | (j << 4) | i;
+        const bool take = pb >= 0 && t[i] == pv && (pv < best || (pv == best && cand < code));
+        best = take ? pv : best;
+        code = take ? cand : code;
+      }
     }
     double mexact = 0.0;
     if (code >= 0) {
@@ -686,9 +762,23 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   for (;;) {
   bool again = false;
   // ---- prologue: v_1 = log2 b - LSE2_r(x_rc + u2_r), a max pass then a sum pass ----
-  for (int r = tid; r < nblk * 32; r += PT) {
-    const double u2 = (r < n && !cold) ? up[r] * kLog2e : 0.0;
-    L.rks[r] = kslot_row(r < n ? L.xn[r] - u2 : __builtin_inf());
+  // (The per-row and per-column values are loaded before any is used; rows
+  // and columns past the end read entry 0, which is not used.)
+  {
+    double uv[RR];
+#pragma unroll
+    for (int t = 0; t < RR; ++t) {
+      const int r = tid + t * PT;
+      uv[t] = up[r < n ? r : 0];
+    }
+#pragma unroll
+    for (int t = 0; t < RR; ++t) {
+      const int r = tid + t * PT;
+      if (r < nblk * 32) {
+        const double u2 = (r < n && !cold) ? uv[t] * kLog2e : 0.0;
+        L.rks[r] = kslot_row(r < n ? L.xn[r] - u2 : __builtin_inf());
+      }
+    }
   }
   set_col_kslots([&](int c) { return c < m ? -L.yn[c] : -__builtin_inf(); });
   __syncthreads();
@@ -696,11 +786,10 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   for (int j = 0; j < NS; ++j)
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[j][i] = -__builtin_inff();
-  for (int blk = blk0; blk < blk1; ++blk) {
-    tiles(blk, row_kslot(blk));
+  rows_pass([&](int) {
 #pragma unroll
     for (int j = 0; j < NS; ++j) acc[j] = __builtin_elementwise_max(acc[j], e[j]);
-  }
+  });
   columns_out(std::integral_constant<bool, true>(), L.sh);
   __syncthreads();
   if constexpr (SPLIT) {
@@ -712,23 +801,28 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   });
 #pragma unroll
   for (int j = 0; j < NS; ++j) acc[j] = f32x16{};
-  for (int blk = blk0; blk < blk1; ++blk) {
-    tiles(blk, row_kslot(blk));
+  rows_pass([&](int) {
 #pragma unroll
     for (int j = 0; j < NS; ++j)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[j][i] += exp2_fast(e[j][i]);
-  }
+  });
   columns_out(std::integral_constant<bool, false>(), L.colx);
   __syncthreads();
   if constexpr (SPLIT) {
     if (!exchange_cols(L.colx, false)) return abandon();
   }
+  double bv[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int c = col_of(j);
+    bv[j] = bp[c < m ? c : 0];
+  }
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
     const int c = col_of(j);
     if (h == 0 && c < m) {
-      const double bj = bp[c];
+      const double bj = bv[j];
       L.bs[c] = bj;
       L.lbs[c] = bj > 0.0 ? log2(bj) : -__builtin_inf();
       const double g = (double)L.sh[c];
@@ -737,14 +831,27 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     }
   }
   set_col_kslots([&](int c) { return c < m ? L.vsh[c] - L.yn[c] : -__builtin_inf(); });
-  for (int r = tid; r < nblk * 32; r += PT) {
-    double c2 = __builtin_inf();
-    if (r < n && L.af[r] > 0.f) {
-      const double u2 = cold ? 0.0 : up[r] * kLog2e;
-      c2 = log2(ap[r]) - u2;
+  {
+    double uv[RR], av[RR];
+#pragma unroll
+    for (int t = 0; t < RR; ++t) {
+      const int r = tid + t * PT;
+      uv[t] = up[r < n ? r : 0];
+      av[t] = ap[r < n ? r : 0];
     }
-    L.c2s[r] = c2;
-    L.rks[r] = kslot_row(r < n ? c2 + L.xn[r] : __builtin_inf());
+#pragma unroll
+    for (int t = 0; t < RR; ++t) {
+      const int r = tid + t * PT;
+      if (r < nblk * 32) {
+        double c2 = __builtin_inf();
+        if (r < n && L.af[r] > 0.f) {
+          const double u2 = cold ? 0.0 : uv[t] * kLog2e;
+          c2 = log2(av[t]) - u2;
+        }
+        L.c2s[r] = c2;
+        L.rks[r] = kslot_row(r < n ? c2 + L.xn[r] : __builtin_inf());
+      }
+    }
   }
   __syncthreads();
 
@@ -904,8 +1011,17 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   __syncthreads();
   {
     const int rend = SPLIT ? (blk1 * 32 < n ? blk1 * 32 : n) : n;
-    for (int i = blk0 * 32 + tid; i < rend; i += PT)
-      up[i] = L.af[i] > 0.f ? (log2(ap[i]) - L.c2s[i]) * kLn2 : -__builtin_inf();
+    double av[RR];
+#pragma unroll
+    for (int t = 0; t < RR; ++t) {
+      const int i = blk0 * 32 + tid + t * PT;
+      av[t] = ap[i < rend ? i : 0];
+    }
+#pragma unroll
+    for (int t = 0; t < RR; ++t) {
+      const int i = blk0 * 32 + tid + t * PT;
+      if (i < rend) up[i] = L.af[i] > 0.f ? (log2(av[t]) - L.c2s[i]) * kLn2 : -__builtin_inf();
+    }
   }
   if (part != 0) return;
   for (int j = tid; j < m; j += PT) vp[j] = L.vsh[j] * kLn2;
@@ -965,7 +1081,18 @@ __device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int 
   const float* Yp = A.Y + A.yoff[p] * d;
   if (!A.max_only) {
     const double* ap = A.a + A.xoff[p];
-    for (int i = tid; i < kPMax; i += PT) L.af[i] = i < n ? (float)ap[i] : 0.f;
+    constexpr int RR = kPMax / PT;
+    double av[RR];
+#pragma unroll
+    for (int t = 0; t < RR; ++t) {
+      const int i = tid + t * PT;
+      av[t] = ap[i < n ? i : 0];
+    }
+#pragma unroll
+    for (int t = 0; t < RR; ++t) {
+      const int i = tid + t * PT;
+      L.af[i] = i < n ? (float)av[t] : 0.f;
+    }
   }
 
   // ---- centre (midpoint of the centroids) and coordinate extent ----
@@ -976,42 +1103,37 @@ __device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int 
     double sx = 0.0, sy = 0.0;
     float lo = __builtin_inff(), hi = -__builtin_inff();
     if (k < d) {
-      // kU points per thread are loaded before any is used (the index past
-      // the end is clamped to the group's first point, i0 < n), so their
-      // latencies overlap; they are summed in the same order as one at a
-      // time.
-      constexpr int kU = 16;
-      for (int i0 = g; i0 < n; i0 += NG * kU) {
-        float xv[kU];
+      // Every point of both clouds this thread reads (kU of each) is loaded
+      // before any is used (indices past the end read point 0; those values
+      // are not used), so all the latencies overlap; they are summed in order
+      // of the point index.
+      constexpr int kU = kPMax / NG;
+      float xv[kU], yv[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const int i = i0 + NG * u;
-          xv[u] = Xp[(int64_t)(i < n ? i : i0) * d + k];
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          if (i0 + NG * u < n) {
-            sx += xv[u];
-            lo = fminf(lo, xv[u]);
-            hi = fmaxf(hi, xv[u]);
-          }
-        }
+      for (int u = 0; u < kU; ++u) {
+        const int i = g + NG * u;
+        xv[u] = Xp[(int64_t)(i < n ? i : 0) * d + k];
       }
-      for (int i0 = g; i0 < m; i0 += NG * kU) {
-        float yv[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const int i = i0 + NG * u;
-          yv[u] = Yp[(int64_t)(i < m ? i : i0) * d + k];
-        }
+      for (int u = 0; u < kU; ++u) {
+        const int i = g + NG * u;
+        yv[u] = Yp[(int64_t)(i < m ? i : 0) * d + k];
+      }
+      // (Selects rather than branches; adding +0 changes a sum only from -0
+      // to +0, which gives the same centre.)
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          if (i0 + NG * u < m) {
-            sy += yv[u];
-            lo = fminf(lo, yv[u]);
-            hi = fmaxf(hi, yv[u]);
-          }
-        }
+      for (int u = 0; u < kU; ++u) {
+        const bool in = g + NG * u < n;
+        sx += in ? (double)xv[u] : 0.0;
+        lo = in ? fminf(lo, xv[u]) : lo;
+        hi = in ? fmaxf(hi, xv[u]) : hi;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const bool in = g + NG * u < m;
+        sy += in ? (double)yv[u] : 0.0;
+        lo = in ? fminf(lo, yv[u]) : lo;
+        hi = in ? fmaxf(hi, yv[u]) : hi;
       }
     }
     scr[tid] = sx;

@@ -391,3 +391,40 @@ def test_points_batch_tables():
     Pe = ragged.points_from_clouds(xs[:1] * 2, ys[:1] * 2, 0.1)
     np.testing.assert_array_equal(Pe.a.cpu().numpy(), np.full(80, 1.0 / 40))
     np.testing.assert_array_equal(Pe.b.cpu().numpy(), np.full(100, 1.0 / 50))
+
+
+def test_points_cost_maximum_shapes_and_ties():
+    """The cost maximum found on the device equals the float64 maximum of
+    ``ot.dist`` (to 1e-12 relative) for sizes from 1 point to 1024, every
+    column-tile count, d from 1 to 16, and clouds whose largest distance is
+    attained by many tied pairs (grid points, duplicated points), through the
+    separate maximum launch and through the solves with 8 and 16 waves."""
+    rng = np.random.default_rng(11)
+    shapes = [(1, 1, 3), (1, 1024, 16), (1024, 1, 16), (33, 31, 1), (64, 257, 7),
+              (300, 520, 16), (1024, 1024, 16), (97, 800, 2), (512, 96, 11), (1000, 1000, 16)]
+    xs, ys = [], []
+    for n, m, d in shapes:
+        xs.append(rng.standard_normal((n, d)).astype(np.float32))
+        ys.append((rng.standard_normal((m, d)) * 0.5 + 1.0).astype(np.float32))
+    # ties: integer grid points, so many pairs share the largest distance
+    g = np.stack(np.meshgrid(*[np.arange(4.0)] * 3), -1).reshape(-1, 3).astype(np.float32)
+    xs.append(g)
+    ys.append(g[::-1].copy())
+    # ties: one cloud is a single point repeated, the other two points repeated
+    xs.append(np.repeat(np.ones((1, 16), np.float32), 700, axis=0))
+    ys.append(np.repeat(np.array([[0.0] * 16, [3.0] * 16], np.float32), 450, axis=0))
+    want = np.array([_cost(x, y, normalize=False).max() for x, y in zip(xs, ys)])
+    for d in sorted({x.shape[1] for x in xs}):
+        idx = [i for i, x in enumerate(xs) if x.shape[1] == d]
+        P = ragged.points_from_clouds([xs[i] for i in idx], [ys[i] for i in idx], 0.1)
+        got = ragged.points_cmax(P).cpu().numpy()
+        torch.cuda.synchronize()
+        for k, i in enumerate(idx):
+            assert abs(got[k] - want[i]) <= 1e-12 * want[i], (shapes + ["grid", "rep"])[i]
+        for waves in (8, 16):
+            res = ragged.sinkhorn_log_ragged_points(P, numItermax=2, stopThr=0.0, waves=waves,
+                                                    split=None)
+            cm = res.cmax.cpu().numpy()
+            for k, i in enumerate(idx):
+                assert abs(cm[k] - want[i]) <= 1e-12 * want[i], (waves, i)
+    print("cost maxima match on", len(xs), "problems", flush=True)
