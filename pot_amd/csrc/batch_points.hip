@@ -149,14 +149,29 @@ __device__ __forceinline__ u32x4 kslot_row(double Q) {
   return k;
 }
 
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
 // Coordinates 8*hh .. 8*hh+7 of the point at p (those past d read
 // coordinate 0 and are not used), loaded ahead of point_ops so that all the
-// loads of a pass are issued before the first is used.
+// loads of a pass are issued before the first is used.  With V4 (d a
+// multiple of 4 and 16-byte aligned clouds) as two 16-byte loads; groups of
+// four past d read coordinates 0-3.
+template <bool V4>
 __device__ __forceinline__ void half_load(const float* p, int d, int hh, float (&v)[8]) {
+  if constexpr (V4) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int kk = 8 * hh + k;
-    v[k] = p[kk < d ? kk : 0];
+    for (int g = 0; g < 2; ++g) {
+      const int k0 = 8 * hh + 4 * g;
+      const f32x4v q = *reinterpret_cast<const f32x4v*>(p + (k0 < d ? k0 : 0));
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[4 * g + e] = q[e];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int kk = 8 * hh + k;
+      v[k] = p[kk < d ? kk : 0];
+    }
   }
 }
 // float -> f16 on v_cvt_f16_f32 (round to nearest even).  The value passes
@@ -164,21 +179,22 @@ __device__ __forceinline__ void half_load(const float* p, int d, int hh, float (
 // double -> float conversion into one double -> f16 conversion, which it
 // would emulate in about 40 integer instructions.
 __device__ __forceinline__ _Float16 f16_of(float x) {
-  asm volatile("" : "+v"(x));
+  asm("" : "+v"(x));
   return (_Float16)x;
 }
 // Half a point (coordinates 8*hh .. 8*hh+7, loaded by half_load) as f16 hi
-// and lo operands of (p - cen) * sc; nrm accumulates the squared norm of the
-// represented (hi + lo) coordinates, mx the largest |coordinate|.
+// and lo operands of (p - cen) * sc, with cn the same coordinates of the
+// centre; nrm accumulates the squared norm of the represented (hi + lo)
+// coordinates, mx the largest |coordinate|.  Straight-line code: coordinates
+// past d and invalid points give 0 by a select.
 __device__ __forceinline__ void point_ops(const float (&pv)[8], bool valid, int d,
-                                          const double* cen, double sc, int hh, u32x4& hi,
+                                          const double (&cn)[8], double sc, int hh, u32x4& hi,
                                           u32x4& lo, double& nrm, float& mx) {
   unsigned hb[8], lb[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int kk = 8 * hh + k;
-    double x = 0.0;
-    if (valid && kk < d) x = ((double)pv[k] - cen[kk]) * sc;
+    const double x = (valid && kk < d) ? ((double)pv[k] - cn[k]) * sc : 0.0;
     const _Float16 xh = f16_of((float)x);
     const double xhd = (double)(float)xh;
     const _Float16 xl = f16_of((float)(x - xhd));
@@ -257,7 +273,6 @@ __device__ __forceinline__ float halves_max(float x) {
   return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
 }
 
-typedef float f32x4v __attribute__((ext_vector_type(4)));
 
 template <int PW>
 struct PointsLds {
@@ -297,6 +312,10 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   const int64_t xo = A.xoff[p], yo = A.yoff[p];
   const float* Xp = A.X + xo * d;
   const float* Yp = A.Y + yo * d;
+  // 16-byte point loads: d a multiple of 4 and both clouds 16-byte aligned
+  // (every row then starts on a 16-byte boundary).
+  const bool v4 = (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(A.X) |
+                                    reinterpret_cast<uintptr_t>(A.Y)) & 15) == 0;
   const double* ap = A.a + xo;
   const double* bp = A.b + yo;
   double* up = A.u + xo;
@@ -405,16 +424,28 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   auto pack = [&](double sc) -> float {
     float mx = 0.f;
     float rv[RQ][8], cv[NS][8];
+    auto load_all = [&](auto v4) {
+      constexpr bool V4 = decltype(v4)::value;
 #pragma unroll
-    for (int u = 0; u < RQ; ++u) {
-      const int q = tid + u * PT, r = (q >> 6) * 32 + (q & 31);
-      half_load(Xp + (int64_t)(r < n ? r : 0) * d, d, (q >> 5) & 1, rv[u]);
-    }
+      for (int u = 0; u < RQ; ++u) {
+        const int q = tid + u * PT, r = (q >> 6) * 32 + (q & 31);
+        half_load<V4>(Xp + (int64_t)(r < n ? r : 0) * d, d, (q >> 5) & 1, rv[u]);
+      }
 #pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      const int c = col_of(j);
-      half_load(Yp + (int64_t)(c < m ? c : 0) * d, d, h, cv[j]);
-    }
+      for (int j = 0; j < NS; ++j) {
+        const int c = col_of(j);
+        half_load<V4>(Yp + (int64_t)(c < m ? c : 0) * d, d, h, cv[j]);
+      }
+    };
+    if (v4)
+      load_all(std::true_type());
+    else
+      load_all(std::false_type());
+    // A lane packs only its half of the coordinates (hh = h for the rows too,
+    // since PT is a multiple of the wave size), so it needs 8 of the centre's.
+    double cn[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cn[k] = L.cen[8 * h + k];
 #pragma unroll
     for (int u = 0; u < RQ; ++u) {
       const int q = tid + u * PT;
@@ -422,7 +453,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
       const int blk = q >> 6, ln = q & 63, r = blk * 32 + (ln & 31), hh = ln >> 5;
       u32x4 hi4, lo4;
       double nh = 0.0;
-      point_ops(rv[u], r < n, d, L.cen, sc, hh, hi4, lo4, nh, mx);
+      point_ops(rv[u], r < n, d, cn, sc, hh, hi4, lo4, nh, mx);
       L.rop[blk][0][ln] = hi4;
       L.rop[blk][1][ln] = lo4;
       nh += __shfl_xor(nh, 32, kWave);
@@ -433,7 +464,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
       const int c = col_of(j);
       u32x4 hi4, lo4;
       double nh = 0.0;
-      point_ops(cv[j], c < m, d, L.cen, sc, h, hi4, lo4, nh, mx);
+      point_ops(cv[j], c < m, d, cn, sc, h, hi4, lo4, nh, mx);
       ahi[j] = hi4;
       alo[j] = lo4;
       nh += __shfl_xor(nh, 32, kWave);
@@ -699,11 +730,25 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
         // All coordinates loaded first (d <= kPD, checked at launch; the
         // index past d is clamped to 0), then summed in order.
         float xs[kPD], ys[kPD];
+        if (v4) {
 #pragma unroll
-        for (int k = 0; k < kPD; ++k) {
-          const int kk = k < d ? k : 0;
-          xs[k] = Xp[(int64_t)r * d + kk];
-          ys[k] = Yp[(int64_t)c * d + kk];
+          for (int g = 0; g < kPD / 4; ++g) {
+            const int k0 = 4 * g < d ? 4 * g : 0;
+            const f32x4v qx = *reinterpret_cast<const f32x4v*>(Xp + (int64_t)r * d + k0);
+            const f32x4v qy = *reinterpret_cast<const f32x4v*>(Yp + (int64_t)c * d + k0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              xs[4 * g + e] = qx[e];
+              ys[4 * g + e] = qy[e];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < kPD; ++k) {
+            const int kk = k < d ? k : 0;
+            xs[k] = Xp[(int64_t)r * d + kk];
+            ys[k] = Yp[(int64_t)c * d + kk];
+          }
         }
 #pragma unroll
         for (int k = 0; k < kPD; ++k) {
