@@ -626,7 +626,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   // Lanes l32 < 16 then store value 4 (l32 >> 2) + (l32 & 3) of the tile.
   auto columns_out = [&](auto max_tag, float* dst) {
     constexpr bool MAX = decltype(max_tag)::value;
-    auto op = [](float x, float y) { return MAX ? fmaxf(x, y) : x + y; };
+    auto op = [](float x, float y) { return MAX ? __builtin_elementwise_maximum(x, y) : x + y; };
     const bool b0 = lane & 1, b1 = (lane >> 1) & 1;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
@@ -670,7 +670,9 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     __syncthreads();
     // Per column tile, each lane keeps the smallest value over the rows and
     // the first block reaching it (the minimum over a block's 16 values is a
-    // min3 chain).  Lanes l and l ^ 32 hold the same row slot (other
+    // v_minimum3 chain: no NaN arises here, padding rows and columns give
+    // +inf, so the IEEE minimum equals fminf without the canonicalising
+    // instructions that fminf costs).  Lanes l and l ^ 32 hold the same row slot (other
     // columns): they agree on one block per tile, the one with the smaller
     // value (the earlier on a tie), whose tile is recomputed (the same MFMA
     // chain, so the same values) to find the column: the first occurrence of
@@ -687,7 +689,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
       for (int j = 0; j < NS; ++j) {
         float mn = e[j][0];
 #pragma unroll
-        for (int i = 1; i < 16; ++i) mn = fminf(mn, e[j][i]);
+        for (int i = 1; i < 16; ++i) mn = __builtin_elementwise_minimum(mn, e[j][i]);
         const bool lt = mn < bestj[j];
         bestj[j] = lt ? mn : bestj[j];
         blkj[j] = lt ? blk : blkj[j];
@@ -831,9 +833,10 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   for (int j = 0; j < NS; ++j)
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[j][i] = -__builtin_inff();
+  // (IEEE maximum, one v_maximum3 per value: padding gives -inf, never NaN.)
   rows_pass([&](int) {
 #pragma unroll
-    for (int j = 0; j < NS; ++j) acc[j] = __builtin_elementwise_max(acc[j], e[j]);
+    for (int j = 0; j < NS; ++j) acc[j] = __builtin_elementwise_maximum(acc[j], e[j]);
   });
   columns_out(std::integral_constant<bool, true>(), L.sh);
   __syncthreads();

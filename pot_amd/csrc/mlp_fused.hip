@@ -12,24 +12,28 @@
 //    per-segment vector rowvec[r / seg_len][256].
 // These are the layers that pot_amd/warmstart/net.py otherwise runs as bf16
 // GEMMs with the same epilogues (bias / broadcast, leaky ReLU, bf16 store),
-// a segment mean and a head kernel.
+// a segment mean and a head kernel; the results agree up to fp32 rounding
+// order (the biases enter first, as the accumulators' starting values) and,
+// in the head, the split of w3 into bf16 hi + lo (2^-16 relative).
 //
 // The input rows are either given (bf16, rows x 32) or built here from the
 // fp32 points (rows x 16), the per-row weights and the per-segment reg and
 // cost maximum, with the arithmetic of the network's input-row kernel:
 // [x * rsqrt(cmax), seg_len * w - 1, (log reg - c) / s, 0 ...].
 //
-// Layout: a 256-thread workgroup (one wave per SIMD) walks whole segments;
-// wave w owns hidden columns 64 w .. 64 w + 63 of both layers and keeps its
-// slices of W1 (32 x 64) and W2 (256 x 64) in registers for the whole
-// launch.  Layer 1 is computed transposed (D[j][r]) so each lane holds four
-// consecutive columns of one row and writes them to LDS with one 8-byte
-// store; layer 2 reads A1 from LDS (16 bytes per lane per MFMA K step).  In
-// mode 0 layer 2 gives D[r][j], so a lane owns a column and the segment sum
-// is a per-lane accumulation; in mode 1 it gives D[j][r] (operands swapped),
-// so a lane owns a row and the head's dot product is per lane.  A1 is double
-// buffered in LDS: one barrier per block (two in mode 1).  The next block's
-// inputs are fetched while the current one is computed.
+// Layout: a 512-thread workgroup (two waves per SIMD, each under 256
+// registers) walks whole segments; wave w owns hidden columns 32 w .. 32 w + 31
+// of both layers and keeps its slices of W1 (32 x 32) and W2 (256 x 32) in
+// registers for the whole launch.  Layer 1 is computed transposed (D[j][r])
+// so each lane holds four consecutive columns of one row and writes them to
+// LDS with one 8-byte store; layer 2 reads A1 from LDS (16 bytes per lane per
+// MFMA K step).  The reductions after layer 2 also run on MFMA, on the
+// packed bf16 A2 values: in mode 0 (D[r][j], a lane owns a column) the
+// segment sum is ones x A2, accumulated over the segment's blocks; in mode 1
+// (D[j][r], operands swapped, a lane owns a row) the head's dot product is
+// w3 x A2 over the wave's columns, the waves' partials then added in LDS.
+// A1 is double buffered in LDS: one barrier per block (two in mode 1).  The
+// next block's inputs are fetched while the current one is computed.
 #include <string.h>
 
 #include "../../include/pot_amd.h"
@@ -44,7 +48,7 @@ constexpr int kFK = 32;              // input width (one K slice pair)
 constexpr int kFD = 16;              // point coordinates
 constexpr int kFRows = 64;           // rows per block
 constexpr int kFStride = kFH + 8;    // bf16 per LDS row (16-byte pad against bank conflicts)
-constexpr int kFW = 4;               // waves per workgroup
+constexpr int kFW = 8;               // waves per workgroup
 constexpr float kFLeak = 0.01f;
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -154,60 +158,69 @@ struct BlockIn<1> {  // built from the points
   }
 };
 
+// bf16 of a float split as hi + lo (each a bf16), packed per pair.
+__device__ __forceinline__ void split_bf2(float x0, float x1, unsigned& hi, unsigned& lo) {
+  const bf16x2 h = {(__bf16)x0, (__bf16)x1};
+  hi = __builtin_bit_cast(unsigned, h);
+  lo = pack_bf2(x0 - (float)h[0], x1 - (float)h[1]);
+}
+
 template <int MODE, int IN>
-__global__ __launch_bounds__(kFW * kWave) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ __launch_bounds__(kFW * kWave) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void mlp_fused_kernel(FusedMlpArgs A) {
   __shared__ __attribute__((aligned(16))) uint16_t act[2][kFRows][kFStride];
   __shared__ float part[2][kFW][kFRows];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int h = lane >> 5, l32 = lane & 31;
-  const int jbase = 64 * w;
+  const int jbase = 32 * w;
 
-  // Weight slices: column j = jbase + 32 c + l32, rows 16 s + 8 h .. +7.
-  u32x4 w1r[2][2], w2r[2][16];
+  // Weight slices: column j = jbase + l32, rows 16 s + 8 h .. +7.
+  u32x4 w1r[2], w2r[16];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
+  for (int s = 0; s < 2; ++s) w1r[s] = load_col8(A.W1, 16 * s + 8 * h, jbase + l32);
 #pragma unroll
-    for (int s = 0; s < 2; ++s) w1r[c][s] = load_col8(A.W1, 16 * s + 8 * h, jbase + 32 * c + l32);
+  for (int s = 0; s < 16; ++s) w2r[s] = load_col8(A.W2, 16 * s + 8 * h, jbase + l32);
+  // The biases enter as the accumulators' starting values.  Layer 2: in mode
+  // 0 every value of a lane is column jbase + l32; in mode 1 value 4 q + i is
+  // column jbase + 8 q + 4 h + i.
+  f32x16 b2init;
 #pragma unroll
-    for (int s = 0; s < 16; ++s) w2r[c][s] = load_col8(A.W2, 16 * s + 8 * h, jbase + 32 * c + l32);
-  }
-  // In mode 0 a lane owns column jbase + 32 c + l32 of layer 2.
-  const float b2c[2] = {A.b2[jbase + l32], A.b2[jbase + 32 + l32]};
-  // Mode 1: the layer-2 bias and the head weights of the lane's values
-  // (column jbase + 32 c + 8 q + 4 h + i), kept in registers: a global load
-  // inside the block loop would be waited for there, with one wave per SIMD.
-  float b2v[2][4][4], w3v[2][4][4];
+  for (int v = 0; v < 16; ++v)
+    b2init[v] = MODE == 0 ? A.b2[jbase + l32] : A.b2[jbase + 8 * (v / 4) + 4 * h + (v % 4)];
+  // Mode 0: the segment sum runs on MFMA against ones; mode 1: the head's dot
+  // product runs on MFMA against w3 split into bf16 hi + lo.  Operand g
+  // (columns jbase + 16 g .. +15) holds, in K slot 8 h + e, column
+  // jbase + 16 g + 4 h + 8 (e / 4) + e % 4: the columns this lane holds.
+  const u32x4 ones4 = {0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u};
+  u32x4 w3hi[2], w3lo[2];
   if constexpr (MODE == 1) {
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int g = 0; g < 2; ++g)
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int j = jbase + 32 * c + 8 * q + 4 * h + i;
-          b2v[c][q][i] = A.b2[j];
-          w3v[c][q][i] = A.w3[j];
-        }
+      for (int e = 0; e < 8; e += 2) {
+        const int j = jbase + 16 * g + 4 * h + 8 * (e / 4) + e % 4;
+        unsigned hi, lo;
+        split_bf2(A.w3[j], A.w3[j + 1], hi, lo);
+        w3hi[g][e / 2] = hi;
+        w3lo[g][e / 2] = lo;
+      }
   }
+  const float b3 = MODE == 1 ? A.b3[0] : 0.f;
 
   const int nseg = A.rows / A.seg_len;
   const int bps = A.seg_len / kFRows;
   BlockIn<IN> nxt;
-  if (blockIdx.x < nseg) nxt.fetch(A, (int64_t)blockIdx.x * A.seg_len, h, l32);
+  if ((int)blockIdx.x < nseg) nxt.fetch(A, (int64_t)blockIdx.x * A.seg_len, h, l32);
   int buf = 0;
   for (int p = blockIdx.x; p < nseg; p += gridDim.x) {
-    float zacc[2] = {0.f, 0.f};
+    f32x16 zsum = {};
     // Per-segment values, once per segment: the layer-1 bias of the lane's
-    // values and (inputs built from the points) the scale and log reg.
+    // values (value 4 q + i: column jbase + 8 q + 4 h + i) and (inputs built
+    // from the points) the scale and log reg.
     const float* bias1 = MODE == 0 ? A.bias1 : A.bias1 + (int64_t)p * kFH;
-    float b1v[2][4][4];
+    f32x16 b1init;
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) b1v[c][q][i] = bias1[jbase + 32 * c + 8 * q + 4 * h + i];
+    for (int v = 0; v < 16; ++v) b1init[v] = bias1[jbase + 8 * (v / 4) + 4 * h + (v % 4)];
     float seg_scale = 1.f, seg_lr = 0.f;
     if constexpr (IN == 1) {
       seg_scale = rsqrtf(A.cmax[p]);
@@ -230,33 +243,21 @@ void mlp_fused_kernel(FusedMlpArgs A) {
       }
       // ---- layer 1, transposed: D[j][r], r = 32 t + l32 ----
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
+      for (int t = 0; t < 2; ++t) {
+        f32x16 acc = mfma16(w1r[0], xin[t][0], b1init);
+        acc = mfma16(w1r[1], xin[t][1], acc);
+        // lane value 4 q + i: column jbase + 8 q + 4 h + i, row 32 t + l32
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          f32x16 acc = {};
-          acc = mfma16(w1r[c][0], xin[t][0], acc);
-          acc = mfma16(w1r[c][1], xin[t][1], acc);
-          // lane value 4 q + i: column jbase + 32 c + 8 q + 4 h + i, row 32 t + l32
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int j0 = jbase + 32 * c + 8 * q + 4 * h;
-            float v[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = leaky(acc[4 * q + i] + b1v[c][q][i]);
-            u32x2 pk;
-            pk[0] = pack_bf2(v[0], v[1]);
-            pk[1] = pack_bf2(v[2], v[3]);
-            *reinterpret_cast<u32x2*>(&act[buf][32 * t + l32][j0]) = pk;
-          }
+        for (int q = 0; q < 4; ++q) {
+          u32x2 pk;
+          pk[0] = pack_bf2(leaky(acc[4 * q]), leaky(acc[4 * q + 1]));
+          pk[1] = pack_bf2(leaky(acc[4 * q + 2]), leaky(acc[4 * q + 3]));
+          *reinterpret_cast<u32x2*>(&act[buf][32 * t + l32][jbase + 8 * q + 4 * h]) = pk;
         }
       }
       __syncthreads();
       // ---- layer 2 ----
-      f32x16 acc[2][2];
-#pragma unroll
-      for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int t = 0; t < 2; ++t) acc[c][t] = f32x16{};
+      f32x16 acc[2] = {b2init, b2init};
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
         u32x4 a[2];
@@ -264,55 +265,54 @@ void mlp_fused_kernel(FusedMlpArgs A) {
         for (int t = 0; t < 2; ++t)
           a[t] = *reinterpret_cast<const u32x4*>(&act[buf][32 * t + l32][16 * s + 8 * h]);
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-          for (int t = 0; t < 2; ++t)
-            acc[c][t] = MODE == 0 ? mfma16(a[t], w2r[c][s], acc[c][t])    // D[r][j]
-                                  : mfma16(w2r[c][s], a[t], acc[c][t]);   // D[j][r]
+        for (int t = 0; t < 2; ++t)
+          acc[t] = MODE == 0 ? mfma16(a[t], w2r[s], acc[t])    // D[r][j]
+                             : mfma16(w2r[s], a[t], acc[t]);   // D[j][r]
       }
+      // A2 = leaky(acc) rounded to bf16, packed per pair: y[t][g] holds
+      // values 8 g .. 8 g + 7 of acc[t].
+      u32x4 y[2][2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            y[t][g][e] = pack_bf2(leaky(acc[t][8 * g + 2 * e]), leaky(acc[t][8 * g + 2 * e + 1]));
       if constexpr (MODE == 0) {
-        // lane value 4 q + i: row 32 t + 8 q + 4 h + i, column jbase + 32 c + l32
+        // Mode 0 (D[r][j], a lane's values all column jbase + l32, rows
+        // 32 t + 8 q + 4 h + i): column sums over the 64 rows, as ones x A2.
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
-          for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) zacc[c] += round_bf(leaky(acc[c][t][v] + b2c[c]));
+          for (int g = 0; g < 2; ++g) zsum = mfma16(ones4, y[t][g], zsum);
       } else {
-        // lane value 4 q + i: column jbase + 32 c + 8 q + 4 h + i, row 32 t + l32
-        float fp[2] = {0.f, 0.f};
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-#pragma unroll
-              for (int t = 0; t < 2; ++t)
-                fp[t] = __builtin_fmaf(round_bf(leaky(acc[c][t][4 * q + i] + b2v[c][q][i])),
-                                       w3v[c][q][i], fp[t]);
-            }
+        // Mode 1 (D[j][r], lane value 4 q + i: column jbase + 8 q + 4 h + i,
+        // row 32 t + l32): the dot product with w3 over the wave's 32
+        // columns, as w3 x A2 (hi and lo parts).
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-          fp[t] += __shfl_xor(fp[t], 32, kWave);
-          if (h == 0) part[buf][w][32 * t + l32] = fp[t];
+          f32x16 d = {};
+#pragma unroll
+          for (int g = 0; g < 2; ++g) {
+            d = mfma16(w3hi[g], y[t][g], d);
+            d = mfma16(w3lo[g], y[t][g], d);
+          }
+          if (h == 0) part[buf][w][32 * t + l32] = d[0];
         }
         __syncthreads();
         if (tid < kFRows) {
-          float f = ((part[buf][0][tid] + part[buf][1][tid]) + part[buf][2][tid]) +
-                    part[buf][3][tid];
-          f += A.b3[0] + (A.qn ? A.qn[r0 + tid] : 0.f);
+          float f = 0.f;
+#pragma unroll
+          for (int ww = 0; ww < kFW; ++ww) f += part[buf][ww][tid];
+          f += b3 + (A.qn ? A.qn[r0 + tid] : 0.f);
           A.out[r0 + tid] = f;
         }
       }
       buf ^= 1;
     }
     if constexpr (MODE == 0) {
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const float z = zacc[c] + __shfl_xor(zacc[c], 32, kWave);
-        if (h == 0) A.out[(int64_t)p * kFH + jbase + 32 * c + l32] = z / (float)A.seg_len;
-      }
+      if (h == 0) A.out[(int64_t)p * kFH + jbase + l32] = zsum[0] / (float)A.seg_len;
     }
   }
 }

@@ -195,7 +195,7 @@ class WarmStartNet:
     def forward_fused(self, Sin, Ein, qn, P, n, m):
         """Inference forward with the fused two-layer kernel (n, m multiples of
         64): the hidden activations stay on chip; same result as ``forward``
-        up to fp32 accumulation order."""
+        up to fp32 rounding order and the head's bf16 hi + lo split of w3."""
         dev = self.device
         w = self.w
         zt = torch.empty((P, H), dtype=torch.float32, device=dev)
