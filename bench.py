@@ -316,11 +316,13 @@ def run_warmstart(args, torch, dev):
                 cm = ragged.points_cmax(R) if warm else None
                 u0 = ws_mod.predict_u0(net, R, X, Y, regs) if warm else None
                 e0.record()
+                # unsolved="nan": no status read-back inside the timed region; a
+                # problem the kernel could not take counts as not converged
                 res = ragged.sinkhorn_log_ragged_points(R, u0=u0, numItermax=args.max_iter,
                                                         stopThr=args.stop,
                                                         check_every=args.check_every,
                                                         fallback_thr=fb_thr, cmax=cm,
-                                                        waves=args.waves)
+                                                        waves=args.waves, unsolved="nan")
             else:
                 R = ragged.from_clouds(X, Y, regs)
                 u0 = ws_mod.predict_u0(net, R, X, Y, regs) if warm else None
@@ -565,7 +567,8 @@ def run_ragged(args, torch, dev):
         elif args.cost == "points":
             R = ragged.points_from_clouds(X, Y, fam.regs)
             e0.record()
-            res = ragged.sinkhorn_log_ragged_points(R, numItermax=2000, stopThr=1e-6)
+            res = ragged.sinkhorn_log_ragged_points(R, numItermax=2000, stopThr=1e-6,
+                                                    unsolved="nan")
             e1.record()
         else:
             R = ragged.from_clouds(X, Y, fam.regs)

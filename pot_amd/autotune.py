@@ -170,7 +170,8 @@ def tune_batch(ns, ms, d=16, cost="points", reg=1e-2, iters=20, reps=2, save=Tru
         P = ragged.points_from_clouds(xs, ys, regs)
 
         def run(cfg):
-            ragged.sinkhorn_log_ragged_points(P, numItermax=iters, stopThr=0.0, waves=cfg[0])
+            ragged.sinkhorn_log_ragged_points(P, numItermax=iters, stopThr=0.0, waves=cfg[0],
+                                              unsolved="nan")
     else:
         R = ragged.from_clouds(xs, ys, regs)
 

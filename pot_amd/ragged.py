@@ -396,13 +396,61 @@ def _split_items(order, parts):
 
 
 def _warn_unsolved(status):
-    """A split problem whose parts gave up waiting for each other reports
-    status 2 with NaN potentials; say so instead of returning NaNs quietly."""
+    """A problem left with status 2 has NaN potentials; say so instead of
+    returning NaNs quietly."""
     bad = int(((status & 3) == 2).sum().item())
     if bad:
-        warnings.warn(f"{bad} problem(s) of the batch were not solved (status 2: cost maximum "
-                      "0, scaled coordinates outside the f16 range, or split parts that gave "
-                      "up waiting for each other); their potentials are NaN")
+        warnings.warn(f"{bad} problem(s) of the batch were not solved (status 2: a cost "
+                      "maximum of 0, whose normalised cost is NaN as in the reference); "
+                      "their potentials are NaN")
+
+
+def _resolve_unsolved(P, res, u0, numItermax, stopThr):
+    """Solve again, one at a time on the float64 dense GPU solver, the problems
+    the point-cloud kernel reported as not solved (status 2): scaled
+    coordinates outside the f16 operand range (small reg on an unnormalised
+    cost), or split parts that gave up waiting for each other.  Each runs the
+    reference's sinkhorn_log on ot.dist(x, y) (divided by its maximum when
+    P.normalize; ot/bregman/_sinkhorn.py:799-918, the marginal checked every
+    10 iterations) from the same start; its potentials, niter, errors and
+    status replace the entry.  A problem whose cost maximum is 0 stays NaN
+    (its normalised cost is 0/0, NaN in the reference too).  Synchronises the
+    host with the stream once, to read the statuses."""
+    st = res.status.cpu().numpy()
+    bad = np.nonzero((st & 3) == 2)[0]
+    if len(bad) == 0:
+        return
+    from . import _solve, bregman
+    d = P.d  # P.X, P.Y: (sum n, d), (sum m, d)
+    aoff = P.aoff.cpu().numpy()
+    boff = P.boff.cpu().numpy()
+    regs = P.regs_host if P.regs_host is not None else P.regs.cpu().numpy()
+    for p in bad:
+        n, m = int(P.ns[p]), int(P.ms[p])
+        a0, b0 = int(aoff[p]), int(boff[p])
+        x = P.X[a0:a0 + n].reshape(1, n, d).to(torch.float64)
+        y = P.Y[b0:b0 + m].reshape(1, m, d).to(torch.float64)
+        M = _solve.sqeuclidean(x, y)[0]
+        mx = float(M.max().item())
+        if P.normalize:
+            if not mx > 0.0:
+                continue
+            M = M / mx
+        ws = None
+        if u0 is not None:
+            ws = (u0[a0:a0 + n].to(torch.float64),
+                  torch.zeros(m, dtype=torch.float64, device=M.device))
+        _, lg = bregman.sinkhorn_log(P.a[a0:a0 + n], P.b[b0:b0 + m], M, float(regs[p]),
+                                     numItermax=numItermax, stopThr=stopThr, log=True,
+                                     warn=False, warmstart=ws)
+        res.u[a0:a0 + n] = torch.as_tensor(lg["log_u"], device=M.device)
+        res.v[b0:b0 + m] = torch.as_tensor(lg["log_v"], device=M.device)
+        errs = [float(e) for e in lg["err"]]
+        res.niter[p] = int(lg["niter"])
+        res.status[p] = 0 if (errs and errs[-1] < stopThr) else 1
+        res.err_first[p] = errs[0] if errs else float("nan")
+        res.err_last[p] = errs[-1] if errs else float("nan")
+        res.cmax[p] = mx if P.normalize else 1.0
 
 
 def _check_parts(parts, ns):
@@ -451,7 +499,7 @@ def _split_workspace(lib, nsplit, dev):
 
 def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=None,
                                fallback_thr=0.0, check_every=10, cmax=None, waves=0,
-                               split="auto"):
+                               split="auto", unsolved="dense"):
     """``sinkhorn_log_ragged`` on point clouds, the cost computed on the fly.
 
     P: a RaggedPoints.  Same loop, stop rule, warm start and fallback as
@@ -463,9 +511,16 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
     default).  ``split``: "auto" (``split_plan``: only batches too small to
     fill the GPU are split), None (one workgroup per problem) or workgroups
     per problem (1..4): a split problem's rows are shared by several
-    workgroups (8 waves each).  Returns a RaggedResult whose ``cmax``
-    holds the cost maxima; asynchronous on the current stream.
+    workgroups (8 waves each).  ``unsolved``: "dense" (default) solves the
+    problems the kernel cannot take (status 2: scaled coordinates outside its
+    f16 operand range, or split parts that gave up waiting) again on the
+    float64 dense solver (``_resolve_unsolved``; reads the statuses back, one
+    host synchronisation); "nan" leaves them NaN with status 2 and returns
+    without synchronising.  Returns a RaggedResult whose ``cmax`` holds the
+    cost maxima.
     """
+    if unsolved not in ("dense", "nan"):
+        raise ValueError(f"unsolved must be 'dense' or 'nan', not {unsolved!r}")
     lib = _lib.load()
     dev = P.device
     B = P.batch
@@ -501,15 +556,17 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
                              cmax, items, torch.from_numpy(table).to(dev), nsplit, sws,
                              numItermax, check_every, stopThr, fallback_thr,
                              ctypes.c_void_p(_device.stream_ptr(dev)), queue, cmax_given=given)
+    else:
+        _launch_points(lib, P, P.X, P.Y, P.regs, u, v, niter, status, e_last, e_first, cmax,
+                       order, numItermax, check_every, stopThr, fallback_thr,
+                       ctypes.c_void_p(_device.stream_ptr(dev)), cmax_given=given, waves=waves,
+                       queue=queue)
+    res = RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
+                       err_last=e_last, cmax=cmax)
+    if unsolved == "dense":
+        _resolve_unsolved(P, res, u0, numItermax, stopThr)
         _warn_unsolved(status)
-        return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
-                            err_last=e_last, cmax=cmax)
-    _launch_points(lib, P, P.X, P.Y, P.regs, u, v, niter, status, e_last, e_first, cmax, order,
-                   numItermax, check_every, stopThr, fallback_thr,
-                   ctypes.c_void_p(_device.stream_ptr(dev)), cmax_given=given, waves=waves,
-                   queue=queue)
-    return RaggedResult(u=u, v=v, niter=niter, status=status, err_first=e_first,
-                        err_last=e_last, cmax=cmax)
+    return res
 
 
 class RaggedGraph:

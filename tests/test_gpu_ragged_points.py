@@ -208,26 +208,41 @@ def test_points_warm_start_fallback_and_edge_cases():
 
 
 def test_points_unnormalised_and_out_of_range_reg():
+    # reg = 1e-8 on an unnormalised cost puts the scaled coordinates outside
+    # the kernel's f16 operand range: with unsolved="nan" the kernel reports
+    # status 2 and NaN potentials; by default that problem is solved again on
+    # the float64 dense solver, as the reference solves it.
     rng = np.random.default_rng(4)
     xs = [rng.standard_normal((50, 4)).astype(np.float32) for _ in range(3)]
     ys = [(rng.standard_normal((60, 4)) + 1.0).astype(np.float32) for _ in range(3)]
-    regs = [2.0, 1e-14, 5.0]
+    regs = [2.0, 1e-8, 5.0]
     P = ragged.points_from_clouds(xs, ys, regs, normalize=False)
+    raw = ragged.sinkhorn_log_ragged_points(P, numItermax=3000, stopThr=1e-7, unsolved="nan")
+    torch.cuda.synchronize()
+    st = raw.status.cpu().numpy()
+    assert st[1] == 2 and (st[[0, 2]] == 0).all()
+    u1, _ = raw.split(P, 1)
+    assert np.isnan(u1.cpu().numpy()).all()
+    with pytest.raises(ValueError):
+        ragged.sinkhorn_log_ragged_points(P, unsolved="skip")
     res = ragged.sinkhorn_log_ragged_points(P, numItermax=3000, stopThr=1e-7)
     torch.cuda.synchronize()
     print("unnormalised batch done", flush=True)
-    st = res.status.cpu().numpy()
-    assert st[1] == 2 and (st[[0, 2]] == 0).all()
-    u1, _ = res.split(P, 1)
-    assert np.isnan(u1.cpu().numpy()).all()
     np.testing.assert_array_equal(res.cmax.cpu().numpy(), [1.0, 1.0, 1.0])
-    for p in (0, 2):
+    for p in (0, 1, 2):
         M = _cost(xs[p], ys[p], normalize=False)
-        Pr = osk.sinkhorn_log(np.full(50, 1 / 50), np.full(60, 1 / 60), M, regs[p],
-                              numItermax=3000, stopThr=1e-7, warn=False)
+        Pr, lr = osk.sinkhorn_log(np.full(50, 1 / 50), np.full(60, 1 / 60), M, regs[p],
+                                  numItermax=3000, stopThr=1e-7, warn=False, log=True)
         u, v = res.split(P, p)
         Pl = _plan(M, u.cpu().numpy(), v.cpu().numpy(), regs[p])
-        assert np.max(np.abs(Pl - Pr)) / Pr.max() < 1e-4, p
+        assert np.max(np.abs(Pl - Pr)) / Pr.max() < (1e-4 if p != 1 else 1e-6), p
+        if p == 1:
+            # the float64 dense solve: the oracle's loop, stop and errors
+            assert int(res.niter[1]) == lr["niter"]
+            assert int(res.status[1]) == (0 if lr["err"][-1] < 1e-7 else 1)
+            np.testing.assert_allclose(float(res.err_last[1]), lr["err"][-1], rtol=1e-6)
+        else:
+            assert int(res.status[p]) == 0
 
 
 def test_points_graph_replays_match_eager():
