@@ -168,20 +168,43 @@ class WarmStartNet:
                                        self._stream())
         _lib.check(rc, "potamd_mlp_fused")
 
+    # Points per cloud from which forward_points' encoders estimate their
+    # segment means (0: every point).  On the 1024-point family, 256 points
+    # cost the start 0.2 iterations (36.04 against 35.85 mean) and cut the
+    # encoders from 2 x 0.65 to about 2 x 0.2 ms per 4096 problems.
+    ENC_POINTS = 256
+
+    def _enc_sample(self, Z, w, P, n):
+        """The points of each cloud the encoder reads, with their per-point
+        weights: all of them when n <= ENC_POINTS, else ENC_POINTS points at
+        stride n // ENC_POINTS, the weights scaled by n / ENC_POINTS so that
+        the input feature n w - 1 keeps its value."""
+        k = int(self.ENC_POINTS)
+        if k <= 0 or n <= k:
+            return Z, w, n
+        s = n // k
+        Zs = Z.view(P, n, DX)[:, :s * k:s].reshape(-1, DX).contiguous()
+        ws = (w.view(P, n)[:, :s * k:s] * (n / k)).reshape(-1).contiguous()
+        return Zs, ws, k
+
     def forward_points(self, X, Y, a, b, regs, cmax, P, n, m):
         """Inference forward straight from the points (n, m multiples of 64): the
         fused kernels build the input rows themselves.  X: (P*n, 16) fp32,
-        a: (P*n,) fp64, regs: (P,) fp64, cmax: (P,) fp32."""
+        a: (P*n,) fp64, regs: (P,) fp64, cmax: (P,) fp32.  The encoders' means
+        come from ``ENC_POINTS`` points of a larger cloud (``_enc_sample``);
+        with every point it is ``forward`` up to fp32 rounding order."""
         dev = self.device
         w = self.w
         X = X.reshape(-1, DX).float().contiguous()
         Y = Y.reshape(-1, DX).float().contiguous()
+        Ye, be, me = self._enc_sample(Y, b, P, m)
         zt = torch.empty((P, H), dtype=torch.float32, device=dev)
-        self._fused(0, None, P * m, m, "W_e1", w["b_e1"], "W_e2", "b_e2", zt, pts=Y, wts=b,
-                    regs=regs, cmax=cmax)
+        self._fused(0, None, P * me, me, "W_e1", w["b_e1"], "W_e2", "b_e2", zt, pts=Ye,
+                    wts=be, regs=regs, cmax=cmax)
+        Xe, ae, ne = self._enc_sample(X, a, P, n)
         zs = torch.empty((P, H), dtype=torch.float32, device=dev)
-        self._fused(0, None, P * n, n, "W_s1", w["b_s1"], "W_s2", "b_s2", zs, pts=X, wts=a,
-                    regs=regs, cmax=cmax)
+        self._fused(0, None, P * ne, ne, "W_s1", w["b_s1"], "W_s2", "b_s2", zs, pts=Xe,
+                    wts=ae, regs=regs, cmax=cmax)
         c = torch.empty((P, H), dtype=torch.float32, device=dev)
         self._op(OP_VECMAT, P=P, k=H, n=H, x=zt, w=w["W_m1z"], y=w["b_m1"], out=c)
         c2 = torch.empty_like(c)

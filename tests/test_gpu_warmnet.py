@@ -263,3 +263,41 @@ def test_fused_forward_many_segments_per_workgroup():
         per = np.linalg.norm(f - g, axis=1) / np.linalg.norm(g, axis=1)
         print(name, "worst per-problem relative difference", per.max(), flush=True)
         assert per.max() < 2e-3, (name, int(per.argmax()), per.max())
+
+
+def test_fused_forward_encoder_subsample():
+    # Clouds of 1024 points made of 256 distinct points, each repeated four
+    # times in a row (weights equal within a group): the encoders' means over
+    # the stride-4 sample equal those over every point, so the prediction with
+    # ENC_POINTS = 256 must equal the one from every point (fp32 rounding
+    # order aside), and that one the per-layer GEMM path.
+    dev = torch.device("cuda")
+    params = wnet.WarmStartNet.init_params(5)
+    rng = np.random.default_rng(21)
+    for k in ("b_e1", "b_e2", "b_s1", "b_s2", "b_m1", "b_m2"):
+        params[k] = rng.standard_normal(params[k].shape) * 0.1
+    params["w_m3"] = rng.standard_normal(params["w_m3"].shape) * 0.05
+    net = wnet.WarmStartNet(params, dev)
+    P, n, m = 6, 1024, 1024
+    base_x = rng.standard_normal((P, n // 4, 16)).astype(np.float32)
+    base_y = (rng.standard_normal((P, m // 4, 16)) + 0.5).astype(np.float32)
+    X = torch.tensor(np.repeat(base_x, 4, axis=1).reshape(-1, 16), device=dev)
+    Y = torch.tensor(np.repeat(base_y, 4, axis=1).reshape(-1, 16), device=dev)
+    wa = np.repeat(rng.dirichlet(np.ones(n // 4), size=P) / 4, 4, axis=1)
+    wb = np.repeat(rng.dirichlet(np.ones(m // 4), size=P) / 4, 4, axis=1)
+    a = torch.tensor(wa.reshape(-1), device=dev)
+    b = torch.tensor(wb.reshape(-1), device=dev)
+    regs = torch.tensor(np.exp(rng.uniform(np.log(5e-3), np.log(5e-2), P)), device=dev)
+    cmax = torch.tensor(rng.uniform(2.0, 8.0, P).astype(np.float32), device=dev)
+    assert net.ENC_POINTS == 256
+    f_sub = net.forward_points(X, Y, a, b, regs, cmax, P, n, m)
+    net.ENC_POINTS = 0
+    f_all = net.forward_points(X, Y, a, b, regs, cmax, P, n, m)
+    Sin, Ein, qn = net.inputs(X, Y, a, b, regs, cmax, P, n, m)
+    f_gemm = net.forward(Sin, Ein, qn, P, n, m, keep=True)
+    torch.cuda.synchronize()
+    e_sub = _rel(f_sub.cpu().numpy(), f_all.cpu().numpy())
+    e_all = _rel(f_all.cpu().numpy(), f_gemm.cpu().numpy())
+    print("stride-4 sample vs every point:", e_sub, "every point vs GEMM path:", e_all, flush=True)
+    assert e_sub < 1e-4, e_sub
+    assert e_all < 2e-3, e_all
