@@ -273,7 +273,6 @@ __device__ __forceinline__ float halves_max(float x) {
   return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
 }
 
-
 template <int PW>
 struct PointsLds {
   u32x4 rop[kPBlocks][2][kWave];  // row operands: [block][hi, lo][lane]

@@ -76,13 +76,11 @@ struct FusedMlpArgs {
   float* out;
 };
 
-__device__ __forceinline__ float bf_to_f(unsigned h) { return __uint_as_float(h << 16); }
 // Two floats to packed bf16 (round to nearest even, v_cvt_pk_bf16_f32).
 __device__ __forceinline__ unsigned pack_bf2(float lo, float hi) {
   const bf16x2 v = {(__bf16)lo, (__bf16)hi};
   return __builtin_bit_cast(unsigned, v);
 }
-__device__ __forceinline__ float round_bf(float x) { return (float)(__bf16)x; }
 // Leaky ReLU with a slope below 1: max(x, slope x) (two VALU operations).
 __device__ __forceinline__ float leaky(float x) { return fmaxf(x, kFLeak * x); }
 
