@@ -163,6 +163,18 @@ __device__ __forceinline__ void split_bf2(float x0, float x1, unsigned& hi, unsi
   lo = pack_bf2(x0 - (float)h[0], x1 - (float)h[1]);
 }
 
+// Head output of row r0 + i: the kFW wave partials of its dot product with w3,
+// plus b3 and the optional per-row offset.
+__device__ __forceinline__ void finish_rows(const FusedMlpArgs& A, const float (*pt)[kFRows],
+                                            float b3, int64_t r0, int i) {
+  const int64_t r = r0 + i;
+  float f = 0.f;
+#pragma unroll
+  for (int ww = 0; ww < kFW; ++ww) f += pt[ww][i];
+  f += b3 + (A.qn ? A.qn[r] : 0.f);
+  A.out[r] = f;
+}
+
 template <int MODE, int IN>
 __global__ __launch_bounds__(kFW * kWave) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void mlp_fused_kernel(FusedMlpArgs A) {
@@ -210,6 +222,9 @@ void mlp_fused_kernel(FusedMlpArgs A) {
   BlockIn<IN> nxt;
   if ((int)blockIdx.x < nseg) nxt.fetch(A, (int64_t)blockIdx.x * A.seg_len, h, l32);
   int buf = 0;
+  // Mode 1: the row block whose wave partials sit in part[buf ^ 1], summed
+  // after the next block's barrier (-1: none), so each block has one barrier.
+  int64_t pend = -1;
   for (int p = blockIdx.x; p < nseg; p += gridDim.x) {
     f32x16 zsum = {};
     // Per-segment values, once per segment: the layer-1 bias of the lane's
@@ -254,6 +269,10 @@ void mlp_fused_kernel(FusedMlpArgs A) {
         }
       }
       __syncthreads();
+      if constexpr (MODE == 1) {
+        if (pend >= 0 && tid < kFRows) finish_rows(A, part[buf ^ 1], b3, pend, tid);
+        pend = r0;
+      }
       // ---- layer 2 ----
       f32x16 acc[2] = {b2init, b2init};
 #pragma unroll
@@ -298,19 +317,17 @@ void mlp_fused_kernel(FusedMlpArgs A) {
           }
           if (h == 0) part[buf][w][32 * t + l32] = d[0];
         }
-        __syncthreads();
-        if (tid < kFRows) {
-          float f = 0.f;
-#pragma unroll
-          for (int ww = 0; ww < kFW; ++ww) f += part[buf][ww][tid];
-          f += b3 + (A.qn ? A.qn[r0 + tid] : 0.f);
-          A.out[r0 + tid] = f;
-        }
       }
       buf ^= 1;
     }
     if constexpr (MODE == 0) {
       if (h == 0) A.out[(int64_t)p * kFH + jbase + l32] = zsum[0] / (float)A.seg_len;
+    }
+  }
+  if constexpr (MODE == 1) {
+    if (pend >= 0) {
+      __syncthreads();
+      if (tid < kFRows) finish_rows(A, part[buf ^ 1], b3, pend, tid);
     }
   }
 }
