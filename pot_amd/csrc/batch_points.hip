@@ -243,6 +243,13 @@ __device__ __forceinline__ int uniform_i(int x) { return __builtin_amdgcn_readfi
 __device__ __forceinline__ float uniform_f(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
+template <typename T>
+__device__ __forceinline__ T* uniform_p(T* x) {
+  const uintptr_t b = reinterpret_cast<uintptr_t>(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return reinterpret_cast<T*>(((uintptr_t)hi << 32) | lo);
+}
 __device__ __forceinline__ double uniform_d(double x) {
   const long long b = __double_as_longlong(x);
   const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
@@ -305,7 +312,9 @@ template <int PW, int NS, bool SPLIT>
 __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>& L, int p, int n,
                                              int m, double ext, int part, int K, char* sws) {
   constexpr int PT = PW * kWave;
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  // (The wave index is uniform: in a scalar register, the addresses and row
+  // ranges built from it need no vector registers across the loops.)
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = uniform_i(tid / kWave);
   const int h = lane >> 5, l32 = lane & 31;
   const int d = A.d;
   const int64_t xo = A.xoff[p], yo = A.yoff[p];
@@ -1091,9 +1100,38 @@ __device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int 
   // Arguments of an out-of-line function count as per-lane values for the
   // compiler: the ones that steer barriers are made visibly uniform.
   PointsArgs A = Ain;
+  // All of them are made so, which keeps the pointers and scalars in scalar
+  // registers instead of vector registers (or scratch) across the solve.
+  A.X = uniform_p(Ain.X);
+  A.Y = uniform_p(Ain.Y);
+  A.xoff = uniform_p(Ain.xoff);
+  A.yoff = uniform_p(Ain.yoff);
+  A.ns = uniform_p(Ain.ns);
+  A.ms = uniform_p(Ain.ms);
+  A.d = uniform_i(Ain.d);
   A.normalize = uniform_i(Ain.normalize);
   A.max_only = uniform_i(Ain.max_only);
+  A.regs = uniform_p(Ain.regs);
+  A.a = uniform_p(Ain.a);
+  A.b = uniform_p(Ain.b);
+  A.u = uniform_p(Ain.u);
+  A.v = uniform_p(Ain.v);
+  A.order = uniform_p(Ain.order);
   A.max_iter = uniform_i(Ain.max_iter);
+  A.check_every = uniform_i(Ain.check_every);
+  A.stop_thr = uniform_d(Ain.stop_thr);
+  A.fallback_thr = uniform_d(Ain.fallback_thr);
+  A.niter = uniform_p(Ain.niter);
+  A.status = uniform_p(Ain.status);
+  A.err_last = uniform_p(Ain.err_last);
+  A.err_first = uniform_p(Ain.err_first);
+  A.cmax = uniform_p(Ain.cmax);
+  A.batch = uniform_i(Ain.batch);
+  A.queue = uniform_p(Ain.queue);
+  A.items = uniform_i(Ain.items);
+  A.split = uniform_p(Ain.split);
+  A.split_ws = uniform_p(Ain.split_ws);
+  A.nsplit = uniform_i(Ain.nsplit);
   item = uniform_i(item);
   // An item is problem p, part `part` of a split problem (part 0 otherwise).
   const int p = item & 0xFFFFFF;
