@@ -285,6 +285,7 @@ struct PointsLds {
   u32x4 rop[kPBlocks][2][kWave];  // row operands: [block][hi, lo][lane]
   u32x4 rks[kPMax];               // row K slots
   double c2s[kPMax];              // row shifts log2 a - u2 (base 2)
+  double la[kPMax];               // log2 a_r (+inf for rows of zero weight)
   double xn[kPMax];               // s |x_r|^2
   double vsh[kPMax];              // column potentials (base 2)
   double yn[kPMax];               // s |y_c|^2
@@ -324,7 +325,6 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   // (every row then starts on a 16-byte boundary).
   const bool v4 = (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(A.X) |
                                     reinterpret_cast<uintptr_t>(A.Y)) & 15) == 0;
-  const double* ap = A.a + xo;
   const double* bp = A.b + yo;
   double* up = A.u + xo;
   double* vp = A.v + yo;
@@ -812,6 +812,26 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   const bool warm = A.fallback_thr > 0.0;
   bool cold = false;
 
+  // b_c and log2 b_c of this lane's columns, once per problem.  (No double
+  // logarithm is left inside the attempt loop below: the compiler would keep
+  // its polynomial's constants in registers across the whole loop.)
+  {
+    double bv[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int c = col_of(j);
+      bv[j] = bp[c < m ? c : 0];
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int c = col_of(j);
+      if (h == 0 && c < m) {
+        L.bs[c] = bv[j];
+        L.lbs[c] = bv[j] > 0.0 ? log2(bv[j]) : -__builtin_inf();
+      }
+    }
+  }
+
   // One attempt from the given start; a rejected warm start repeats the
   // prologue and the iterations once from the cold start.
   for (;;) {
@@ -868,32 +888,25 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   if constexpr (SPLIT) {
     if (!exchange_cols(L.colx, false)) return abandon();
   }
-  double bv[NS];
-#pragma unroll
-  for (int j = 0; j < NS; ++j) {
-    const int c = col_of(j);
-    bv[j] = bp[c < m ? c : 0];
-  }
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
     const int c = col_of(j);
     if (h == 0 && c < m) {
-      const double bj = bv[j];
-      L.bs[c] = bj;
-      L.lbs[c] = bj > 0.0 ? log2(bj) : -__builtin_inf();
+      // (log2 of the column sum on the fp32 log, as in the iterations: a
+      // finite shift g makes the sum at least about 1.)
       const double g = (double)L.sh[c];
-      L.vsh[c] = (bj > 0.0 && __builtin_isfinite(g)) ? log2(bj) - (g + log2((double)L.colx[c]))
-                                                     : -__builtin_inf();
+      L.vsh[c] = (L.bs[c] > 0.0 && __builtin_isfinite(g))
+                     ? L.lbs[c] - (g + (double)__builtin_amdgcn_logf(L.colx[c]))
+                     : -__builtin_inf();
     }
   }
   set_col_kslots([&](int c) { return c < m ? L.vsh[c] - L.yn[c] : -__builtin_inf(); });
   {
-    double uv[RR], av[RR];
+    double uv[RR];
 #pragma unroll
     for (int t = 0; t < RR; ++t) {
       const int r = tid + t * PT;
       uv[t] = up[r < n ? r : 0];
-      av[t] = ap[r < n ? r : 0];
     }
 #pragma unroll
     for (int t = 0; t < RR; ++t) {
@@ -902,7 +915,7 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
         double c2 = __builtin_inf();
         if (r < n && L.af[r] > 0.f) {
           const double u2 = cold ? 0.0 : uv[t] * kLog2e;
-          c2 = log2(av[t]) - u2;
+          c2 = L.la[r] - u2;
         }
         L.c2s[r] = c2;
         L.rks[r] = kslot_row(r < n ? c2 + L.xn[r] : __builtin_inf());
@@ -1067,16 +1080,10 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   __syncthreads();
   {
     const int rend = SPLIT ? (blk1 * 32 < n ? blk1 * 32 : n) : n;
-    double av[RR];
 #pragma unroll
     for (int t = 0; t < RR; ++t) {
       const int i = blk0 * 32 + tid + t * PT;
-      av[t] = ap[i < rend ? i : 0];
-    }
-#pragma unroll
-    for (int t = 0; t < RR; ++t) {
-      const int i = blk0 * 32 + tid + t * PT;
-      if (i < rend) up[i] = L.af[i] > 0.f ? (log2(av[t]) - L.c2s[i]) * kLn2 : -__builtin_inf();
+      if (i < rend) up[i] = L.af[i] > 0.f ? (L.la[i] - L.c2s[i]) * kLn2 : -__builtin_inf();
     }
   }
   if (part != 0) return;
@@ -1176,7 +1183,9 @@ __device__ __attribute__((noinline)) void one_problem(const PointsArgs Ain, int 
 #pragma unroll
     for (int t = 0; t < RR; ++t) {
       const int i = tid + t * PT;
-      L.af[i] = i < n ? (float)av[t] : 0.f;
+      const float af = i < n ? (float)av[t] : 0.f;
+      L.af[i] = af;
+      L.la[i] = af > 0.f ? log2(av[t]) : __builtin_inf();
     }
   }
 
