@@ -222,20 +222,6 @@ __device__ __forceinline__ int lane_now() {
   return l;
 }
 
-// Sum over the wave of a double, the shuffle addresses rebuilt from
-// lane_now() (the same butterfly, in the same order, as wave_sum).
-__device__ __forceinline__ double wave_sum_now(double x) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    const int src = (lane_now() ^ o) << 2;
-    const long long b = __double_as_longlong(x);
-    const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(src, (int)b);
-    const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(src, (int)(b >> 32));
-    x += __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-  }
-  return x;
-}
-
 // Values that every lane of the workgroup holds (read back from LDS after a
 // barrier), made visibly uniform to the compiler so that the control flow
 // they steer -- around barriers -- is uniform.
@@ -1000,45 +986,74 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     }
 
     // ---- row potentials, marginal error, stop decision ----
+    // (Every LDS value of a section is read before any is used -- indexes
+    // clamped, results selected -- so each section waits for LDS once.)
     const int rend = SPLIT ? (blk1 * 32 < n ? blk1 * 32 : n) : n;
-    for (int r = blk0 * 32 + wave * kWave + lane_now(); r < rend; r += PT) {
-      if (L.af[r] > 0.f) {
-        // sh[r] lies in [2^-60, 2^60] (the safe range of the sweep).
-        const double c = L.c2s[r] + (double)__builtin_amdgcn_logf(L.sh[r]);
-        L.c2s[r] = c;
-        L.rks[r] = kslot_row(c + L.xn[r]);
-      }
-    }
-    double e2 = 0.0;
+    {
+      const int r0 = blk0 * 32 + wave * kWave + lane_now();
+      float afv[RR], shv[RR];
+      double c2v[RR], xnv[RR];
 #pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      const int c = col_now(j);
-      if (half_now() == 0 && c < m) {
-        const double dd = (double)L.colx[c] - L.bs[c];
-        e2 += dd * dd;
+      for (int t = 0; t < RR; ++t) {
+        const int r = r0 + t * PT;
+        const int rr = r < rend ? r : 0;
+        afv[t] = L.af[rr];
+        shv[t] = L.sh[rr];
+        c2v[t] = L.c2s[rr];
+        xnv[t] = L.xn[rr];
+      }
+#pragma unroll
+      for (int t = 0; t < RR; ++t) {
+        const int r = r0 + t * PT;
+        if (r < rend && afv[t] > 0.f) {
+          // sh[r] lies in [2^-60, 2^60] (the safe range of the sweep).
+          const double c = c2v[t] + (double)__builtin_amdgcn_logf(shv[t]);
+          L.c2s[r] = c;
+          L.rks[r] = kslot_row(c + xnv[t]);
+        }
       }
     }
-    e2 = wave_sum_now(e2);
-    if (lane == 0) L.red[wave] = e2;
+    float csv[NS];
+    {
+      double bsv[NS];
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        const int c = col_now(j);
+        const int cc = c < m ? c : 0;
+        csv[j] = L.colx[cc];
+        bsv[j] = L.bs[cc];
+      }
+      double e2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        const double dd = (double)csv[j] - bsv[j];
+        e2 += (half_now() == 0 && col_now(j) < m) ? dd * dd : 0.0;
+      }
+      e2 = wave_sum_dpp_d(e2);
+      if (lane == 0) L.red[wave] = e2;
+    }
     __syncthreads();
-    if (tid == 0) {
+    // Every thread forms the same error from the PW wave sums (added in wave
+    // order), so the decision needs no second barrier.
+    int fl = 0;
+    {
+      double rv[PW];
+#pragma unroll
+      for (int w = 0; w < PW; ++w) rv[w] = L.red[w];
       double t = 0.0;
 #pragma unroll
-      for (int w = 0; w < PW; ++w) t += L.red[w];
-      const double err = sqrt(t);
-      int stop = 0;
+      for (int w = 0; w < PW; ++w) t += rv[w];
+      const double err = sqrt(uniform_d(t));
       if (ii % A.check_every == 0) {
         err_last = err;
         if (ii == 0) {
           err_first = err;
-          if (warm && !restarted && err > A.fallback_thr) stop = 2;
+          if (warm && !restarted && err > A.fallback_thr) fl = 2;
         }
-        if (stop == 0 && err < A.stop_thr) stop = 1;
+        if (fl == 0 && err < A.stop_thr) fl = 1;
       }
-      L.flag = stop;
+      fl = uniform_i(fl);
     }
-    __syncthreads();
-    const int fl = uniform_i(L.flag);
     if (fl == 2) {
       // Warm start rejected: restart from the cold start u0 = 0.
       restarted = 1;
@@ -1054,20 +1069,32 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
     }
     if (ii == A.max_iter - 1) break;
     // ---- column update v_c += log2 b_c - log2 colsum_c (own columns) ----
+    {
+      double lbv[NS], vsv[NS], ynv[NS];
 #pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      const int c = col_now(j);
-      if (half_now() == 0 && c < m) {
-        // log2 of the column sum on the fp32 log (the fp32 sums carry no more
-        // than that); a subnormal sum is scaled into the normal range first
-        const float cs = L.colx[c];
-        const bool tiny = cs < 1.1754944e-38f;
-        const double lcs = (double)__builtin_amdgcn_logf(tiny ? cs * 0x1p64f : cs) -
-                           (tiny ? 64.0 : 0.0);
-        const double vn = L.lbs[c] > -__builtin_inf() ? L.vsh[c] + L.lbs[c] - lcs
+      for (int j = 0; j < NS; ++j) {
+        const int c = col_now(j);
+        const int cc = c < m ? c : 0;
+        lbv[j] = L.lbs[cc];
+        vsv[j] = L.vsh[cc];
+        ynv[j] = L.yn[cc];
+      }
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        const int c = col_now(j);
+        if (half_now() == 0 && c < m) {
+          // log2 of the column sum on the fp32 log (the fp32 sums carry no
+          // more than that); a subnormal sum is scaled into the normal range
+          // first
+          const float cs = csv[j];
+          const bool tiny = cs < 1.1754944e-38f;
+          const double lcs = (double)__builtin_amdgcn_logf(tiny ? cs * 0x1p64f : cs) -
+                             (tiny ? 64.0 : 0.0);
+          const double vn = lbv[j] > -__builtin_inf() ? vsv[j] + lbv[j] - lcs
                                                       : -__builtin_inf();
-        L.vsh[c] = vn;
-        ak[j] = kslot_col(vn - L.yn[c]);
+          L.vsh[c] = vn;
+          ak[j] = kslot_col(vn - ynv[j]);
+        }
       }
     }
     __syncthreads();
