@@ -238,7 +238,7 @@ int potamd_sinkhorn_log_batch_points(int batch, int max_n, int max_m, int d, con
  *    k); the parts of a split problem must be taken together, so list them
  *    next to each other.
  * split: device int[B], 0 for a whole problem, else parts | slot << 4 with
- *    parts 2..4 (each part needs at least one 32-row block: n >= 32 parts)
+ *    parts 2..8 (each part needs at least one 32-row block: n >= 32 parts)
  *    and slot < nsplit, one slot per split problem.
  * split_ws: nsplit exchange slots (potamd_points_split_workspace_size bytes,
  *    256-byte aligned), zero before the first launch; the kernel leaves the

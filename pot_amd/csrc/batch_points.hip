@@ -91,10 +91,10 @@ struct PointsArgs {
   int nsplit;            // number of slots
 };
 
-// Split problems: the parts of problem p (2 or 4 workgroups, each owning a
+// Split problems: the parts of problem p (2 to 8 workgroups, each owning a
 // contiguous range of row blocks) combine their column reductions through
 // one exchange slot: a counter, then double-buffered per-part columns.
-constexpr int kSplitMax = 4;
+constexpr int kSplitMax = 8;
 constexpr size_t kSplitCols = 256;                                    // byte offset
 constexpr size_t kSplitDbl = kSplitCols + 2ull * kSplitMax * kPMax * 4;  // byte offset
 constexpr size_t kSplitSlot = kSplitDbl + 256;

@@ -216,7 +216,7 @@ hipError_t launch_gather_final(int B, int n, int m, const int* pstate, double* c
 // Point-cloud batch solver (batch_points.hip).  The split exchange slots
 // are kPointsSplitSlot bytes each; a slot's counters live in its first
 // kPointsSplitCounters bytes.
-constexpr size_t kPointsSplitSlot = 33280;
+constexpr size_t kPointsSplitSlot = 66048;
 constexpr size_t kPointsSplitCounters = 256;
 int points_validate(int batch, int max_n, int max_m, int d, const float* X, const float* Y,
                     const int64_t* xoff, const int64_t* yoff, const int* ns, const int* ms,

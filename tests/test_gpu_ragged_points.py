@@ -301,16 +301,17 @@ def test_batch_autotuner_picks_a_compiled_config(tmp_path, monkeypatch):
 
 
 def test_split_problems_fixed_iterations_match_oracle():
-    # Problems shared by 2 and 4 workgroups (uneven n, m, d < 16), against the
-    # float64 oracle at a fixed iteration count; the cost maxima are combined
-    # across the parts.
+    # Problems shared by 2, 4 and 8 workgroups (uneven n, m, d < 16), against
+    # the float64 oracle at a fixed iteration count; the cost maxima are
+    # combined across the parts.
     fam = problems.ragged_family(6, seed=77, n_range=(130, 700), d=5, kmin=1, kmax=3,
                                  reg_range=(0.02, 0.2))
     P = ragged.points_from_clouds(fam.xs, fam.ys, fam.regs)
     ns = np.array([len(x) for x in fam.xs])
     parts = np.where(ns >= 128, 4, 1)
     parts[::3] = np.where(ns[::3] >= 64, 2, 1)
-    assert (parts > 1).sum() >= 4
+    parts[4:] = np.where(ns[4:] >= 256, 8, parts[4:])  # 487 and 694 rows: 8 parts
+    assert (parts == 8).sum() == 2 and (parts > 1).sum() >= 4
     it = 40
     res = ragged.sinkhorn_log_ragged_points(P, numItermax=it, stopThr=0.0, split=parts)
     torch.cuda.synchronize()

@@ -131,6 +131,37 @@ def test_batch_tuner_key_and_lookup(monkeypatch):
     assert autotune.lookup_batch("points", [], [], 16) is None
 
 
+def test_split_plan_tiers_and_budget():
+    import numpy as np
+
+    from pot_amd import ragged
+    # small batches of large problems: 8 parts each while they fit the CUs
+    big = np.full(16, 1024)
+    assert (ragged.split_plan(big, big, cus=256) == 8).all()
+    # 64 of them: 8-part problems fall back to 4 before anyone gives up parts
+    big = np.full(64, 1024)
+    assert (ragged.split_plan(big, big, cus=256) == 4).all()
+    # size tiers and their row minima
+    ns = np.array([1024, 600, 300, 200, 120, 60, 1024, 250])
+    ms = np.array([1024, 700, 600, 700, 4000, 4000, 100, 4000])
+    np.testing.assert_array_equal(ragged.split_plan(ns, ms, cus=256),
+                                  [8, 4, 2, 2, 2, 1, 1, 4])
+    # over budget: the smallest split problems give up parts first; with regs
+    # the size is weighed as n m / reg
+    ns = np.array([512, 512, 640])
+    ms = np.array([512, 512, 640])
+    np.testing.assert_array_equal(ragged.split_plan(ns, ms, cus=8), [2, 2, 4])
+    np.testing.assert_array_equal(ragged.split_plan(ns, ms, cus=6), [1, 2, 4])
+    np.testing.assert_array_equal(ragged.split_plan(ns, ms, cus=6, regs=[0.01, 0.1, 0.1]),
+                                  [2, 1, 4])
+    # batches of more than 2 problems per CU are not split
+    assert (ragged.split_plan(np.full(9, 1024), np.full(9, 1024), cus=4) == 1).all()
+    with pytest.raises(ValueError):
+        ragged._check_parts([9], [1024])
+    with pytest.raises(ValueError):
+        ragged._check_parts([8], [255])
+
+
 def test_points_and_network_argument_validation():
     # invalid arguments are rejected before any device call
     lib = _lib.load()
@@ -173,7 +204,7 @@ def test_noalloc_rejects_more_slots_than_its_scratch():
     # checked on the host before any GPU work: nsplit above the fixed maximum
     lib = _lib.load()
     P = ctypes.c_void_p(0x1000)
-    assert lib.potamd_noalloc_scratch_bytes() >= 256 + 256 * 33280
+    assert lib.potamd_noalloc_scratch_bytes() >= 256 + 256 * 66048
     for nsplit, want in ((257, -4), (-1, -1)):
         rc = lib.potamd_sinkhorn_log_batch_points_split_noalloc(
             1, 64, 64, 16, P, P, P, P, P, P, P, 1, P, P, P, P, 10, 10, 1e-6, 0.0, 1, P, P,
