@@ -610,6 +610,24 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
       body(blk);
     }
   };
+  // The same pass on the hi parts only (x_hi . y_hi plus the potentials):
+  // half the matrix work, exponents within about 2^-11 |X| |Y| of the full
+  // ones -- enough for a shift, which only has to be near the maximum.
+  auto rows_pass_hi = [&](auto&& body) {
+    u32x4 nbh = L.rop[blk0][0][lane], nbk = row_kslot(blk0);
+    for (int blk = blk0; blk < blk1; ++blk) {
+      const u32x4 bh = nbh, bk = nbk;
+      const int nb = blk + 1 < blk1 ? blk + 1 : blk;
+      nbh = L.rop[nb][0][lane];
+      nbk = row_kslot(nb);
+      const f32x16 z = {};
+#pragma unroll
+      for (int j = 0; j < NS; ++j) e[j] = mfma_h(ahi[j], bh, z);
+#pragma unroll
+      for (int j = 0; j < NS; ++j) e[j] = mfma_b(ak[j], bk, e[j]);
+      body(blk);
+    }
+  };
   // Per-column reduction of acc over the rows (sum or max) into dst.
   f32x16 acc[NS];
   // Recursive halving: an exchange with lane ^ 1 pairs values i and i+1
@@ -847,8 +865,10 @@ __device__ __forceinline__ void solve_points(const PointsArgs& A, PointsLds<PW>&
   for (int j = 0; j < NS; ++j)
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[j][i] = -__builtin_inff();
-  // (IEEE maximum, one v_maximum3 per value: padding gives -inf, never NaN.)
-  rows_pass([&](int) {
+  // (IEEE maximum, one v_maximum3 per value: padding gives -inf, never NaN.
+  // The maxima only shift the sum pass below, whose log-sum-exp is exact for
+  // any shift near the maximum, so the pass runs on the hi parts.)
+  rows_pass_hi([&](int) {
 #pragma unroll
     for (int j = 0; j < NS; ++j) acc[j] = __builtin_elementwise_maximum(acc[j], e[j]);
   });

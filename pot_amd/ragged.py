@@ -359,7 +359,9 @@ def split_plan(ns, ms, cus=None, regs=None):
     columns: about 3x the speed of one workgroup on 8 CUs, against 2.5x on
     4), smallest first, then the smallest split problems give up parts.
     ``regs`` (optional): sizes are then weighed as n m / reg, a proxy for the
-    iterations a problem needs.  With the split problems queued first
+    iterations a problem needs (the automatic split leaves it out: a graph
+    is planned before it sees the regs, and an eager solve of the same batch
+    must split alike).  With the split problems queued first
     (``_split_items``), every part is taken by the first wave of workgroups,
     so no part waits for a partner that a busy workgroup has not dequeued
     yet."""
@@ -558,8 +560,11 @@ def sinkhorn_log_ragged_points(P, u0=None, numItermax=1000, stopThr=1e-9, order=
     queue = torch.zeros(2, dtype=torch.int32, device=dev)
     parts = None
     if split is not None:
-        parts = split_plan(P.ns, P.ms, regs=P.regs_host) \
-            if (isinstance(split, str) and split == "auto") else _check_parts(split, P.ns)
+        # (sizes only, as RaggedGraph plans before it sees the regs, so the
+        # eager solve and a graph replay of one batch split alike and agree
+        # bit for bit)
+        parts = split_plan(P.ns, P.ms) if (isinstance(split, str) and split == "auto") \
+            else _check_parts(split, P.ns)
     if parts is not None and (parts > 1).any():
         table, nsplit = _split_tables(parts)
         items = torch.from_numpy(_split_items(order.cpu().numpy(), parts)).to(dev)
