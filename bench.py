@@ -40,6 +40,14 @@ HBM_PEAK_GBS = 8000.0
 # device copy (FETCH_SIZE 524315 KiB: x 2.000; WRITE_SIZE 1048576 KiB: x 1.000).
 DENSE_C2_TRAFFIC = (540584.37 + 8429.00 + 2.66) * 1024 * 2.000 \
     + (16911.19 + 136.00 + 0.04) * 1024 * 1.000
+# HBM bytes of one cold launch of the point-cloud solver on the 4096-problem
+# family (rocprofv3 --pmc, the same scaling): fetch 853253.25 KiB x 2.000,
+# write 1136088.28 KiB x 1.000.  The points are read three times (centring,
+# operand packing for the cost-maximum search, packing for the solve); the
+# writes are mostly the once-per-problem register saves of the out-of-line
+# per-problem function.  Against 0.70 GB of algorithmic bytes (points, weights,
+# start and potentials once), at about 94 GB/s: far from bounding the kernel.
+POINTS_COLD_TRAFFIC = 853253.25 * 1024 * 2.000 + 1136088.28 * 1024 * 1.000
 MFMA_PEAK_TFLOPS = {"f32": 157.3, "f16": 2500.0, "bf16": 2500.0, "bf16x3": 2500.0, "fp8": 5000.0, "f64": 78.6}
 
 
@@ -396,7 +404,7 @@ def run_warmstart(args, torch, dev):
         valu_peak = 256 * 4 * 64 / 14.6 * 2.4e9
         roof = {"bound": "mfma", "kernel": "batch_points_kernel (cold launch)",
                 "achieved": tf, "peak": MFMA_PEAK_TFLOPS["f16"], "unit": "TFLOP/s",
-                "frac": tf / MFMA_PEAK_TFLOPS["f16"], "traffic": None,
+                "frac": tf / MFMA_PEAK_TFLOPS["f16"], "traffic": POINTS_COLD_TRAFFIC,
                 "alg_flops_per_launch": alg_flops,
                 "flops_per_element_iteration": 4 * d,
                 "avg_launch_ms": cold["kernel_ms"],
@@ -408,8 +416,9 @@ def run_warmstart(args, torch, dev):
                                "frac": cold["sweep_elements"] / secs / valu_peak},
                 "note": "achieved = algorithmic x.y^T flops (4 d per element-iteration) over "
                         "the cold launch; the points stay in LDS and registers, so HBM "
-                        "carries only points and potentials (traffic null); the kernel is "
-                        "bound by the VALU (exp2 + add + fma per element), see valu_bound"}
+                        "carries only points, weights, potentials and register saves "
+                        "(traffic: measured bytes per launch, about 0.09 TB/s); the kernel "
+                        "is bound by the VALU (exp2 + add + fma per element), see valu_bound"}
     else:
         roof = {"bound": "hbm", "kernel": "batch_log_kernel (cold launch)",
                 "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
