@@ -312,24 +312,28 @@ def points_from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None
         regs_h = np.full(B, regs_h[0])
     if regs_h.size != B:
         raise ValueError(f"regs has {regs_h.size} entries for {B} problems")
-    # The host-built tables (offsets, sizes, regs, and the launch order, which
+    # The host-built tables (offsets, regs, sizes, and the launch order, which
     # an upload at solve time would make wait for the work already queued on
-    # the stream, such as a warm start's network) go up in one copy.
-    host = np.empty(6 * B, dtype=np.int64)
-    host[0:B] = aoff
-    host[B:2 * B] = boff
-    host[2 * B:3 * B] = ns
-    host[3 * B:4 * B] = ms
-    host[4 * B:5 * B] = regs_h.view(np.int64)
-    host[5 * B:6 * B] = _points_order(ns, ms, regs_h)
-    tab = torch.from_numpy(host).to(device)
+    # the stream, such as a warm start's network) go up in one copy of one
+    # byte buffer, viewed on the device in their own types (no conversion
+    # kernels).
+    raw = np.empty(3 * B * 8 + 3 * B * 4, dtype=np.uint8)
+    h64 = raw[:24 * B].view(np.int64)
+    h64[0:B] = aoff
+    h64[B:2 * B] = boff
+    h64[2 * B:3 * B] = regs_h.view(np.int64)
+    h32 = raw[24 * B:].view(np.int32)
+    h32[0:B] = ns
+    h32[B:2 * B] = ms
+    h32[2 * B:3 * B] = _points_order(ns, ms, regs_h)
+    tab = torch.from_numpy(raw).to(device)
+    t64 = tab[:24 * B].view(torch.int64)
+    t32 = tab[24 * B:].view(torch.int32)
     return RaggedPoints(device=device, ns=ns, ms=ms, d=d, X=_pack_points(xs, device),
-                        Y=_pack_points(ys, device), aoff=tab[0:B], boff=tab[B:2 * B],
-                        ns_d=tab[2 * B:3 * B].to(torch.int32),
-                        ms_d=tab[3 * B:4 * B].to(torch.int32), a=_weights(a, ns, device),
-                        b=_weights(b, ms, device), regs=tab[4 * B:5 * B].view(torch.float64),
-                        normalize=bool(normalize), regs_host=regs_h,
-                        order=tab[5 * B:6 * B].to(torch.int32))
+                        Y=_pack_points(ys, device), aoff=t64[0:B], boff=t64[B:2 * B],
+                        ns_d=t32[0:B], ms_d=t32[B:2 * B], a=_weights(a, ns, device),
+                        b=_weights(b, ms, device), regs=t64[2 * B:3 * B].view(torch.float64),
+                        normalize=bool(normalize), regs_host=regs_h, order=t32[2 * B:3 * B])
 
 
 def _check_order(order, B):
