@@ -290,8 +290,8 @@ def points_from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None
     """
     device = device or _device.require_gpu()
     if isinstance(xs, torch.Tensor) and isinstance(ys, torch.Tensor):
-        ns = [int(xs.shape[1])] * int(xs.shape[0])
-        ms = [int(ys.shape[1])] * int(ys.shape[0])
+        ns = np.full(int(xs.shape[0]), int(xs.shape[1]), dtype=np.int64)
+        ms = np.full(int(ys.shape[0]), int(ys.shape[1]), dtype=np.int64)
     else:
         ns = [len(x) for x in xs]
         ms = [len(y) for y in ys]
