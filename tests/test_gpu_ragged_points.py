@@ -407,6 +407,18 @@ def test_points_batch_tables():
     Pe = ragged.points_from_clouds(xs[:1] * 2, ys[:1] * 2, 0.1)
     np.testing.assert_array_equal(Pe.a.cpu().numpy(), np.full(80, 1.0 / 40))
     np.testing.assert_array_equal(Pe.b.cpu().numpy(), np.full(100, 1.0 / 50))
+    # (B, n, d) tensors: the same tables, each in its own type
+    Xt = torch.tensor(np.stack([xs[0], xs[0]]), device="cuda")
+    Yt = torch.tensor(np.stack([ys[0], ys[0], ys[0]])[:2], device="cuda")
+    Pt = ragged.points_from_clouds(Xt, Yt, [0.2, 0.1])
+    np.testing.assert_array_equal(Pt.aoff.cpu().numpy(), [0, 40])
+    np.testing.assert_array_equal(Pt.boff.cpu().numpy(), [0, 50])
+    np.testing.assert_array_equal(Pt.ns_d.cpu().numpy(), [40, 40])
+    np.testing.assert_array_equal(Pt.ms_d.cpu().numpy(), [50, 50])
+    np.testing.assert_array_equal(Pt.regs.cpu().numpy(), [0.2, 0.1])
+    np.testing.assert_array_equal(Pt.order.cpu().numpy(), [1, 0])
+    assert Pt.aoff.dtype == torch.int64 and Pt.regs.dtype == torch.float64
+    assert Pt.ms_d.dtype == torch.int32 and Pt.order.dtype == torch.int32
 
 
 def test_points_cost_maximum_shapes_and_ties():
