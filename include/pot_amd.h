@@ -424,6 +424,31 @@ int potamd_online_plan(int dtype, const void* X, int64_t ldx, const void* Y, int
                        const double* a, const double* b, double* P, int64_t ldp, double* sums,
                        void* workspace, size_t ws_bytes, void* stream);
 
+/* ---- Greedy Sinkhorn -------------------------------------------------- */
+
+/* Workspace of potamd_greenkhorn: K = exp(-M / reg) and its transpose in
+ * fp64 (16 n m bytes) and the two violation vectors; 0 outside the supported
+ * sizes (n, m <= 2097120). */
+size_t potamd_greenkhorn_workspace_size(int n, int m);
+
+/* ot.bregman.greenkhorn (ot/bregman/_sinkhorn.py:921-1102): the greedy loop
+ * of :1058-1086 (argmax of |viol| and |viol_2|, first index on ties, NaN
+ * largest; the larger one's row or column scaling updated, the violations
+ * updated incrementally; stop = max of the two |violations| read before the
+ * update, stop <= stop_thr ends the loop after it), in fp64 whatever the
+ * cost's dtype, in one launch.
+ * u, v: fp64 device scalings, in: the start (1/n, 1/m, or exp of a warm
+ *    start), out: the final ones.
+ * P: fp64 device n x m, or NULL: the plan u_i K_ij v_j of the final scalings.
+ * info (host, 4 ints): [0] the reference's `ii` at exit (-1 when
+ *    max_iter < 1), [1] 1 when the stop rule ended the loop, [2] status.
+ * Returns POTAMD_CONVERGED, or POTAMD_MAX_ITER (the reference warns).
+ * Synchronises the stream. */
+int potamd_greenkhorn(int dtype, const void* M, int64_t ldm, int n, int m, double reg,
+                      const double* a, const double* b, double* u, double* v, int max_iter,
+                      double stop_thr, double* P, void* workspace, size_t ws_bytes, void* stream,
+                      int* info);
+
 /* ---- Log-stabilised Sinkhorn ------------------------------------------ */
 
 /* Workspace of potamd_sinkhorn_stabilized (holds K in the cost's dtype). */

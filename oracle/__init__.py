@@ -12,6 +12,7 @@ Modules
 -------
 ``oracle.sinkhorn``  ``sinkhorn``, ``sinkhorn2``, ``sinkhorn_knopp``, ``sinkhorn_log``
                      (ot/bregman/_sinkhorn.py:21-918)
+``oracle.greenkhorn`` ``greenkhorn`` (ot/bregman/_sinkhorn.py:921-1102)
 ``oracle.batch``     ``ot.batch`` linear solvers (ot/batch/_linear.py, ot/batch/_utils.py)
 ``oracle.utils``     ``dist``, ``euclidean_distances``, ``unif``, ``dist0``,
                      ``make_1D_gauss`` (ot/utils.py, ot/datasets.py)

@@ -4,7 +4,8 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
 Follows, step by step:
 
-* ``sinkhorn``        ot/bregman/_sinkhorn.py:21-234   (method dispatch)
+* ``sinkhorn``        ot/bregman/_sinkhorn.py:21-234   (method dispatch; greenkhorn
+  is ``oracle.greenkhorn``)
 * ``sinkhorn2``       ot/bregman/_sinkhorn.py:237-484  (loss variant)
 * ``sinkhorn_knopp``  ot/bregman/_sinkhorn.py:487-687  (scaling form)
 * ``sinkhorn_log``    ot/bregman/_sinkhorn.py:690-918  (log-domain form)
@@ -62,8 +63,13 @@ def sinkhorn(a, b, M, reg, method="sinkhorn", numItermax=1000, stopThr=1e-9,
         return sinkhorn_log(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
                             verbose=verbose, log=log, warn=warn,
                             warmstart=warmstart, **kwargs)
-    if name in ("greenkhorn", "sinkhorn_stabilized", "sinkhorn_epsilon_scaling"):
-        raise NotImplementedError("oracle covers sinkhorn / sinkhorn_log only")
+    if name == "greenkhorn":
+        from .greenkhorn import greenkhorn
+        return greenkhorn(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
+                          verbose=verbose, log=log, warn=warn, warmstart=warmstart)
+    if name in ("sinkhorn_stabilized", "sinkhorn_epsilon_scaling"):
+        raise NotImplementedError("oracle.sinkhorn covers sinkhorn / sinkhorn_log / greenkhorn; "
+                                  "see oracle.stabilized")
     raise ValueError("Unknown method '%s'." % method)
 
 

@@ -102,6 +102,11 @@ SIGNATURES = {
     "potamd_online_plan": (c_int, [
         c_int, c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int, c_dbl, c_dbl,
         c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_size, c_vp]),
+    "potamd_greenkhorn_workspace_size": (c_size, [c_int, c_int]),
+    "potamd_greenkhorn": (c_int, [
+        c_int, c_vp, c_i64, c_int, c_int, c_dbl,   # dtype, M, ldm, n, m, reg
+        c_vp, c_vp, c_vp, c_vp, c_int, c_dbl,      # a, b, u, v, max_iter, stop_thr
+        c_vp, c_vp, c_size, c_vp, c_ip]),          # P, workspace, ws_bytes, stream, info
     "potamd_stabilized_workspace_size": (c_size, [c_int, c_int, c_int]),
     "potamd_sinkhorn_stabilized": (c_int, [
         c_int, c_vp, c_i64, c_int, c_int, c_dbl,   # dtype, M, ldm, n, m, reg

@@ -7,18 +7,20 @@ ot/bregman/_sinkhorn.py:21-918; the iterations run in the gfx950 HIP library.
 * ``sinkhorn2``       _sinkhorn.py:237-484  loss variant
 * ``sinkhorn_knopp``  _sinkhorn.py:487-687  scaling form
 * ``sinkhorn_log``    _sinkhorn.py:690-918  log-domain form
+* ``greenkhorn``      _sinkhorn.py:921-1102 greedy form
 
-Methods the build does not implement (``greenkhorn``, ``sinkhorn_stabilized``,
-``sinkhorn_epsilon_scaling``) raise ``NotImplementedError``; unknown methods
-raise ``ValueError`` like the reference (:233-234).
+``sinkhorn_stabilized`` and ``sinkhorn_epsilon_scaling`` live in
+``pot_amd.stabilized``; unknown methods raise ``ValueError`` like the
+reference (:233-234).
 """
 
+import ctypes
 import warnings
 
 import numpy as np
 import torch
 
-from . import _lib, _solve
+from . import _device, _lib, _solve
 from ._device import Ingress, is_empty
 from ._grad import envelope, requires_grad
 from .utils import list_to_array
@@ -28,7 +30,6 @@ NOT_CONVERGED = (
     "increase the number of iterations `numItermax` "
     "or the regularization parameter `reg`."
 )
-_UNBUILT = ("greenkhorn",)
 
 
 def _print_errs(errs, start=0, step=10):
@@ -42,10 +43,6 @@ def _print_errs(errs, start=0, step=10):
 
 
 def _method_error(method):
-    if method.lower() in _UNBUILT:
-        return NotImplementedError(
-            "method '%s' is not part of the pot_amd hot path; "
-            "use 'sinkhorn' or 'sinkhorn_log'" % method)
     return ValueError("Unknown method '%s'." % method)
 
 
@@ -61,6 +58,9 @@ def sinkhorn(a, b, M, reg, method="sinkhorn", numItermax=1000, stopThr=1e-9,
         return sinkhorn_log(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
                             verbose=verbose, log=log, warn=warn,
                             warmstart=warmstart, **kwargs)
+    if name == "greenkhorn":
+        return greenkhorn(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
+                          verbose=verbose, log=log, warn=warn, warmstart=warmstart)
     if name == "sinkhorn_stabilized":
         from .stabilized import sinkhorn_stabilized
         return sinkhorn_stabilized(a, b, M, reg, numItermax=numItermax, stopThr=stopThr,
@@ -123,6 +123,54 @@ def _prep(a, b, M):
     bt = torch.full((m,), 1.0 / m, dtype=torch.float64, device=ing.device) \
         if is_empty(b) else ing.dev(b, torch.float64)
     return ing, Mt, at, bt
+
+
+def greenkhorn(a, b, M, reg, numItermax=10000, stopThr=1e-9, verbose=False, log=False,
+               warn=True, warmstart=None):
+    """Greedy Sinkhorn (_sinkhorn.py:921-1102), the whole loop in one GPU launch.
+
+    Each iteration updates the one row or column scaling whose marginal
+    violates its weight the most (the first index on a tie), keeps both
+    violation vectors up to date incrementally, and stops once the largest
+    violation read before the update is ``<= stopThr`` (after that update, as
+    the reference).  Computed in float64 whatever the cost's dtype; returns
+    the plan ``u_i K_ij v_j`` in M's dtype, and with ``log`` the dict
+    ``{"u", "v", "n_iter"}`` (scalings, not their logarithms).  ``verbose`` is
+    accepted and, as in the reference, prints nothing.
+    """
+    ing, Mt, at, bt = _prep(a, b, M)
+    n, m = Mt.shape
+    if bt.dim() > 1:
+        raise ValueError("greenkhorn takes one target histogram b (the reference's loop "
+                         "indexes b as a vector)")
+    dev = ing.device
+    if warmstart is None:
+        u = torch.full((n,), 1.0 / n, dtype=torch.float64, device=dev)
+        v = torch.full((m,), 1.0 / m, dtype=torch.float64, device=dev)
+    else:
+        u = torch.exp(ing.dev(warmstart[0], torch.float64).reshape(n))
+        v = torch.exp(ing.dev(warmstart[1], torch.float64).reshape(m))
+    lib = _lib.load()
+    code = _lib.F32 if Mt.dtype == torch.float32 else _lib.F64
+    ws_bytes = lib.potamd_greenkhorn_workspace_size(n, m)
+    if ws_bytes == 0:
+        raise ValueError(f"greenkhorn: unsupported size {n} x {m}")
+    ws = _device.workspace(ws_bytes, dev)
+    P = torch.empty((n, m), dtype=torch.float64, device=dev)
+    info = (ctypes.c_int * 4)()
+    p = _solve._ptr
+    rc = lib.potamd_greenkhorn(code, p(Mt), Mt.stride(0), n, m, float(reg), p(at), p(bt), p(u),
+                               p(v), int(numItermax), float(stopThr), p(P), p(ws), ws_bytes,
+                               ctypes.c_void_p(_device.stream_ptr(dev)), info)
+    _lib.check(rc, "potamd_greenkhorn")
+    if rc == _lib.MAX_ITER and warn:
+        warnings.warn(NOT_CONVERGED)
+    G = ing.out(P)
+    if log:
+        if info[0] < 0:
+            raise UnboundLocalError("numItermax must be positive when log=True")
+        return G, {"u": ing.out(u), "v": ing.out(v), "n_iter": int(info[0])}
+    return G
 
 
 def sinkhorn_log(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,

@@ -10,8 +10,10 @@ GPU path with the reference's inputs and tolerances.
                            (any start converges to the same plan);
 * :118-127                 sinkhorn_log and sinkhorn_stabilized (tau = 1) agree
                            at reg = 1e-5 (loss atol 1e-6);
-* :42-74                   stopThr = 0, numItermax = 1 warns for every method;
-                           greenkhorn is outside this build and raises.
+* :42-74                   stopThr = 0, numItermax = 1 warns for every method.
+
+greenkhorn joins the cases the reference runs it in (:42-74, :228-314,
+:330-387); the warm-start case (:1450-1486) does not list it.
 """
 
 import warnings
@@ -25,6 +27,7 @@ import pot_amd
 pytestmark = pytest.mark.gpu
 
 METHODS = ["sinkhorn", "sinkhorn_log", "sinkhorn_stabilized", "sinkhorn_epsilon_scaling"]
+ALL_METHODS = METHODS + ["greenkhorn"]
 
 
 def _gauss_1d(n, m, s):
@@ -51,7 +54,7 @@ def _problem100(seed=0):
 def test_every_method_gives_the_same_plan():
     u, M = _problem100()
     plans = {}
-    for method in METHODS:
+    for method in ALL_METHODS:
         plans[method] = pot_amd.sinkhorn(u, u, M, 1, method=method, stopThr=1e-10)
         print(method, "done", flush=True)
     ub = torch.tensor(u, device="cuda")
@@ -61,6 +64,7 @@ def test_every_method_gives_the_same_plan():
     np.testing.assert_allclose(G0, plans["sinkhorn_log"], atol=1e-5)
     np.testing.assert_allclose(G0, plans["sinkhorn_stabilized"], atol=1e-5)
     np.testing.assert_allclose(G0, plans["sinkhorn_epsilon_scaling"], atol=1e-5)
+    np.testing.assert_allclose(G0, plans["greenkhorn"], atol=1e-5)
     np.testing.assert_allclose(G0, Gt, atol=1e-5)
 
 
@@ -71,8 +75,12 @@ def test_empty_weights_default_to_uniform():
         print(method, "done", flush=True)
         np.testing.assert_allclose(u, G.sum(1), atol=1e-5)
         np.testing.assert_allclose(u, G.sum(0), atol=1e-5)
-    with pytest.raises(NotImplementedError):
-        pot_amd.sinkhorn([], [], M, 1, method="greenkhorn", stopThr=1e-10, log=True)
+    # greenkhorn: the reference only runs it (its 1000 single-row updates are
+    # not held to the marginals); here the plan must be the uniform one's
+    G, log = pot_amd.sinkhorn([], [], M, 1, method="greenkhorn", stopThr=1e-10, log=True)
+    Gu = pot_amd.sinkhorn(u, u, M, 1, method="greenkhorn", stopThr=1e-10)
+    np.testing.assert_array_equal(G, Gu)
+    assert log["n_iter"] == 999
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
@@ -83,7 +91,7 @@ def test_outputs_keep_dtype_and_device(dtype):
     M = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)
     ub = torch.tensor(u, dtype=dtype, device="cuda")
     Mb = torch.tensor(M, dtype=dtype, device="cuda")
-    for method in METHODS:
+    for method in ALL_METHODS:
         Gb = pot_amd.sinkhorn(ub, ub, Mb, 1, method=method, stopThr=1e-10)
         assert isinstance(Gb, torch.Tensor) and Gb.dtype == dtype and Gb.device == Mb.device
     for method in ("sinkhorn", "sinkhorn_stabilized", "sinkhorn_log"):
@@ -123,7 +131,7 @@ def test_log_and_stabilized_agree_at_small_reg():
     np.testing.assert_allclose(loss1, loss2, atol=1e-6)
 
 
-@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("method", ALL_METHODS)
 def test_convergence_warning_every_method(method):
     n = 100
     a1 = _gauss_1d(n, m=30, s=10)

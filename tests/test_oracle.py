@@ -508,3 +508,90 @@ def test_stabilized_multi_histogram_oracle_fixed_point():
     l2 = sinkhorn_stabilized_multi([.5, .5], np.array([[.5, .5], [.5, .5]]),
                                    [[0., 1.], [1., 0.]], 1, stopThr=1e-14)
     np.testing.assert_allclose(l2, [0.26894142] * 2, atol=1e-8)  # 1 / (1 + e)
+
+
+# ------------------------------------------------------------- greenkhorn
+
+def test_greenkhorn_doctest_value():
+    import oracle.greenkhorn as ogk
+    # ot/bregman/_sinkhorn.py:995-1001, printed to 8 digits
+    G = ogk.greenkhorn([0.5, 0.5], [0.5, 0.5], [[0.0, 1.0], [1.0, 0.0]], 1)
+    np.testing.assert_allclose(G, [[0.36552929, 0.13447071], [0.13447071, 0.36552929]],
+                               atol=5e-9)
+
+
+def test_greenkhorn_converges_to_the_sinkhorn_plan():
+    # Both solve the same entropic problem: at a tight stop the plans agree.
+    import oracle.greenkhorn as ogk
+    rng = np.random.default_rng(4)
+    x, y = rng.normal(size=(12, 2)), rng.normal(size=(15, 2)) + 0.5
+    M = ou.dist(x, y)
+    a = rng.dirichlet(np.ones(12))
+    b = rng.dirichlet(np.ones(15))
+    G, lg = ogk.greenkhorn(a, b, M, 0.5, numItermax=200000, stopThr=1e-13, log=True)
+    P = osk.sinkhorn_log(a, b, M, 0.5, numItermax=10000, stopThr=1e-13)
+    assert lg["n_iter"] < 200000 - 1
+    np.testing.assert_allclose(G, P, atol=1e-12)
+    np.testing.assert_allclose(G, lg["u"][:, None] * np.exp(-M / 0.5) * lg["v"][None, :],
+                               rtol=1e-12)
+
+
+def test_greenkhorn_violations_and_greedy_choice():
+    # The incrementally kept violations equal the recomputed marginals of u K v
+    # minus the weights after every update; each update moves the row or
+    # column of largest |violation| (the first index on a tie) and leaves its
+    # own marginal equal to its weight.
+    import oracle.greenkhorn as ogk
+    rng = np.random.default_rng(9)
+    M = rng.uniform(0.0, 1.0, size=(7, 5))
+    a = rng.dirichlet(np.ones(7))
+    b = rng.dirichlet(np.ones(5))
+    K = np.exp(-M / 0.3)
+    trace = []
+    ogk.greenkhorn(a, b, M, 0.3, numItermax=60, stopThr=0.0, warn=False, _trace=trace)
+    assert len(trace) == 60
+    u_prev = np.full(7, 1 / 7)
+    v_prev = np.full(5, 1 / 5)
+    for ii, row, idx, u, v, viol, viol_2 in trace:
+        # the choice, from the marginals before this update
+        G0 = u_prev[:, None] * K * v_prev[None, :]
+        r0, c0 = np.abs(G0.sum(1) - a), np.abs(G0.sum(0) - b)
+        if row:
+            assert idx == int(np.argmax(r0)) and r0.max() > c0.max()
+        else:
+            assert idx == int(np.argmax(c0)) and r0.max() <= c0.max()
+        G = u[:, None] * K * v[None, :]
+        np.testing.assert_allclose(viol, G.sum(1) - a, atol=1e-15)
+        np.testing.assert_allclose(viol_2, G.sum(0) - b, atol=1e-15)
+        if row:
+            assert abs(G[idx].sum() - a[idx]) < 1e-15
+            np.testing.assert_array_equal(v, v_prev)
+        else:
+            assert abs(G[:, idx].sum() - b[idx]) < 1e-15
+            np.testing.assert_array_equal(u, u_prev)
+        u_prev, v_prev = u, v
+
+
+def test_greenkhorn_stop_after_the_update_and_warnings():
+    # The stop value is read before the update, the update still happens, and
+    # the loop ends at that iteration: with a huge stopThr exactly one update
+    # (n_iter 0), no warning; with numItermax reached, a warning and
+    # n_iter = numItermax - 1.
+    import oracle.greenkhorn as ogk
+    rng = np.random.default_rng(2)
+    M = rng.uniform(size=(4, 6))
+    a, b = np.full(4, 0.25), np.full(6, 1 / 6)
+    trace = []
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        G, lg = ogk.greenkhorn(a, b, M, 1.0, stopThr=10.0, log=True, _trace=trace)
+    assert lg["n_iter"] == 0 and len(trace) == 1
+    _, _, _, u, v, _, _ = trace[0]
+    np.testing.assert_allclose(G, u[:, None] * np.exp(-M) * v[None, :], rtol=1e-15)
+    with pytest.warns(UserWarning, match="did not converge"):
+        _, lg = ogk.greenkhorn(a, b, M, 1.0, numItermax=3, stopThr=0.0, log=True)
+    assert lg["n_iter"] == 2
+    # empty weights are uniform (_sinkhorn.py:1034-1037)
+    G = ogk.greenkhorn([], [], M, 1.0, stopThr=1e-12, numItermax=100000)
+    np.testing.assert_allclose(G.sum(1), a, atol=1e-12)
+    np.testing.assert_allclose(G.sum(0), b, atol=1e-12)
