@@ -372,4 +372,6 @@ def sinkhorn_knopp(a, b, M, reg, numItermax=1000, stopThr=1e-9, verbose=False,
 
 
 # ot.bregman also exposes the stabilised solvers (ot/bregman/_sinkhorn.py:1105, :1371)
+# and the lazy empirical ones (ot/bregman/__init__.py:37-43)
 from .stabilized import sinkhorn_epsilon_scaling, sinkhorn_stabilized  # noqa: E402
+from .empirical import empirical_sinkhorn, empirical_sinkhorn2  # noqa: E402

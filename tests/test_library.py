@@ -233,3 +233,14 @@ def test_lazy_tensor_indexing_semantics():
     assert T[::10, 5].shape == (10,)
     with pytest.raises(NotImplementedError):
         T["error"]
+
+
+def test_bregman_namespace_has_the_path_functions():
+    # the names ot.bregman exports for this path (ot/bregman/__init__.py:13-43)
+    import pot_amd.bregman as b
+    for name in ("sinkhorn", "sinkhorn2", "sinkhorn_knopp", "sinkhorn_log", "greenkhorn",
+                 "sinkhorn_stabilized", "sinkhorn_epsilon_scaling", "empirical_sinkhorn",
+                 "empirical_sinkhorn2"):
+        assert callable(getattr(b, name)), name
+    with pytest.raises(ValueError):
+        b.sinkhorn([0.5, 0.5], [0.5, 0.5], [[0.0, 1.0], [1.0, 0.0]], 1.0, method="unknown")
