@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void gk_build_kernel(const TM* M, int64_t ldm,
 template <int THREADS>
 __device__ __forceinline__ double block_sum(double x, double* red) {
   constexpr int W = THREADS / kWave;
-  x = wave_sum(x);
+  x = wave_sum_dpp_d(x);
   if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = x;
   __syncthreads();
   double s = red[0];
@@ -145,16 +145,40 @@ __device__ __forceinline__ bool gk_above(double x, int i, double bx, int bi) {
   return x > bx || (x == bx && i < bi);
 }
 
+// The same order through DPP (no LDS round trips): the largest key -- the
+// bits of a non-negative double order like its value, a NaN above +inf, and
+// a lane with no candidate (-inf) counts as +0 with index INT_MAX -- then
+// the smallest index holding it.  Every lane gets the same pair.
+template <int CTRL>
+__device__ __forceinline__ unsigned long long gk_dpp_u64(unsigned long long x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, 0xF, 0xF, false);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ __forceinline__ unsigned long long gk_readlane_u64(unsigned long long x, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
 __device__ __forceinline__ void gk_wave_argmax(double& bx, int& bi) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    const double x = __shfl_xor(bx, o, kWave);
-    const int i = __shfl_xor(bi, o, kWave);
-    if (gk_above(x, i, bx, bi)) {
-      bx = x;
-      bi = i;
-    }
-  }
+  const unsigned long long key =
+      (bx >= 0.0 || __builtin_isnan(bx)) ? (unsigned long long)__double_as_longlong(bx) : 0ull;
+  unsigned long long k = key;
+  k = max(k, gk_dpp_u64<0xB1>(k));   // quad_perm [1,0,3,2]
+  k = max(k, gk_dpp_u64<0x4E>(k));   // quad_perm [2,3,0,1]
+  k = max(k, gk_dpp_u64<0x141>(k));  // row_half_mirror
+  k = max(k, gk_dpp_u64<0x140>(k));  // row_mirror
+  k = max(max(gk_readlane_u64(k, 0), gk_readlane_u64(k, 16)),
+          max(gk_readlane_u64(k, 32), gk_readlane_u64(k, 48)));
+  int i = key == k ? bi : INT_MAX;
+  i = min(i, __builtin_amdgcn_update_dpp(0, i, 0xB1, 0xF, 0xF, false));
+  i = min(i, __builtin_amdgcn_update_dpp(0, i, 0x4E, 0xF, 0xF, false));
+  i = min(i, __builtin_amdgcn_update_dpp(0, i, 0x141, 0xF, 0xF, false));
+  i = min(i, __builtin_amdgcn_update_dpp(0, i, 0x140, 0xF, 0xF, false));
+  i = min(min(__builtin_amdgcn_readlane(i, 0), __builtin_amdgcn_readlane(i, 16)),
+          min(__builtin_amdgcn_readlane(i, 32), __builtin_amdgcn_readlane(i, 48)));
+  bx = __longlong_as_double((long long)k);
+  bi = i;
 }
 
 // The loop.  C > 0: u, v and both violation vectors live in LDS (n, m <= C)
