@@ -18,7 +18,8 @@ d = 64 problem with the cost computed on the fly, 200 fixed iterations, one
 ``--precision`` of the x . y^T term), ``precision`` (n = m = 65536, d = 128,
 reg = 5e-2, 100 iterations on the online cost in every precision against the
 full-size fp64 GPU run, and on a 4096-point subsample against the float64
-NumPy oracle), ``copy`` (1 GiB device copy, the HBM reference).
+NumPy oracle), ``copy`` (1 GiB device copy, the HBM reference),
+``greenkhorn`` (the greedy Sinkhorn at n = m = 2048: 10000 updates, reg 0.05).
 
 Prints one JSON line; only N = 1 is supported.
 """
@@ -56,7 +57,7 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="warmstart", choices=["warmstart", "dense", "c1", "ragged", "copy",
+    ap.add_argument("--workload", default="warmstart", choices=["warmstart", "dense", "c1", "ragged", "copy", "greenkhorn",
                              "online", "precision"])
     ap.add_argument("--precision", default="bf16x3",
                     choices=["f32", "bf16", "bf16x3", "fp8", "f64"])
@@ -787,6 +788,53 @@ def run_precision(args, torch, dev):
             "wall_s": time.perf_counter() - t_all}
 
 
+def run_greenkhorn(args, torch, dev):
+    """The greedy Sinkhorn (ot.bregman.greenkhorn) through the drop-in: n = m =
+    2048 points in d = 2 (y shifted by 1), cost / max, uniform weights, reg
+    0.05, 10000 updates (stopThr 0).  A step is one whole call (K, K^T, the
+    violations, the loop and the plan) on inputs resident in HBM.  With the
+    CPU leg, 1000 updates of the float64 NumPy oracle on the same problem."""
+    from pot_amd import bregman
+    rng = np.random.default_rng(0)
+    n, reg, its = 2048, 0.05, 10000
+    x = rng.normal(size=(n, 2))
+    y = rng.normal(size=(n, 2)) + 1.0
+    M = ((x[:, None, :] - y[None, :, :]) ** 2).sum(-1)
+    M /= M.max()
+    a = np.full(n, 1.0 / n)
+    at, Mt = torch.tensor(a, device=dev), torch.tensor(M, device=dev)
+
+    def solve():
+        return bregman.greenkhorn(at, at, Mt, reg, numItermax=its, stopThr=0.0, warn=False)
+
+    for _ in range(args.warmup):
+        solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solve()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    out = {"value": its / dt, "unit": "updates/s", "ms_per_step": dt * 1e3, "dtype": "f64",
+           "config": {"workload": f"greenkhorn n=m={n} d=2 reg={reg} {its} updates"},
+           "roofline": {"bound": "hbm", "kernel": "gk_loop_kernel (whole call)",
+                        "achieved": its * n * 8 / dt / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": its * n * 8 / dt / 1e9 / HBM_PEAK_GBS,
+                        "traffic": None,
+                        "note": "one row of K (n doubles) per update; the loop is bound by "
+                                "the latency of its reductions and barriers, not by HBM"}}
+    if not args.no_cpu:
+        from oracle.greenkhorn import greenkhorn
+        k = 1000
+        t0 = time.perf_counter()
+        greenkhorn(a, a, M, reg, numItermax=k, stopThr=0.0, warn=False)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k / t, "unit": "updates/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} updates of the float64 NumPy greenkhorn on the "
+                                         "same problem"}
+    return out
+
+
 def run_copy(args, torch, dev):
     """Device-to-device copy of 1 GiB: the HBM reference of the stream kernels and
     the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -824,7 +872,7 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda", 0)
     runner = {"warmstart": run_warmstart, "dense": run_dense, "c1": run_c1, "ragged": run_ragged,
-              "copy": run_copy, "online": run_online,
+              "copy": run_copy, "online": run_online, "greenkhorn": run_greenkhorn,
               "precision": run_precision}[args.workload]
     res = runner(args, torch, dev)
     # The default workload carries the headline metric; the others name
@@ -836,6 +884,7 @@ def main():
         "copy": "device copy GB/s",
         "online": "sinkhorn_log iterations/s, online cost (never stored)",
         "precision": "online-cost precision study",
+        "greenkhorn": "greenkhorn greedy updates/s, 2048^2 float64",
     }[args.workload]
     line = {"metric": metric, "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
