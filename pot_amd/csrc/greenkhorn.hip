@@ -20,10 +20,11 @@
 // equals that product of the final u and v.
 //
 // The loop is sequential by construction (each update moves the next argmax),
-// so one workgroup of 1024 threads runs it: per iteration one pass over the
-// violations for both argmaxes, then one or two passes over a row of K (or
-// K^T) with block reductions, 4 barriers in all.  Up to 4096 points a side
-// the scalings and violations stay in LDS for the whole loop.
+// so one workgroup runs it (512 threads up to 2048 points a side, 1024
+// above): per iteration one pass over the violations for both argmaxes, then
+// two passes over a row of K (or K^T) with block reductions, 2 barriers in
+// all.  Up to 4096 points a side the scalings and violations stay in LDS for
+// the whole loop.
 #include <limits.h>
 #include <math.h>
 
@@ -34,10 +35,9 @@
 namespace potamd {
 namespace {
 
-constexpr int kGT = 1024;             // threads of the loop workgroup
-constexpr int kGW = kGT / kWave;      // its waves
 constexpr int kGridMax = 65535 * 32;  // rows (or columns) the tile grid covers
 constexpr int kGLds = 4096;           // n, m up to this: the loop's state in LDS (128 KB)
+constexpr int kGSmall = 2048;         // n, m up to this: a 512-thread loop (64 KB of LDS)
 
 struct GkOut {
   int niter;      // the reference's ii at exit
@@ -105,6 +105,17 @@ __device__ __forceinline__ double block_sum(double x, double* red) {
 #pragma unroll
   for (int w = 1; w < W; ++w) s += red[w];
   return s;
+}
+
+// The same sum with the per-wave partials added on DPP by one wave's lanes
+// (the order differs from block_sum's; every thread gets the same value).
+template <int W>
+__device__ __forceinline__ double block_sum_dpp(double x, double* red) {
+  const int lane = threadIdx.x & (kWave - 1);
+  x = wave_sum_dpp_d(x);
+  if (lane == 0) red[threadIdx.x / kWave] = x;
+  __syncthreads();
+  return wave_sum_dpp_d(lane < W ? red[lane] : 0.0);
 }
 
 // The starting violations: block r < n sums row r of G = (u K) v (minus
@@ -186,28 +197,41 @@ __device__ __forceinline__ void gk_wave_argmax(double& bx, int& bi) {
 // registers between the two passes over it, so an iteration touches HBM only
 // to read that row once; C == 0: the same loop with the state in HBM, for
 // larger problems.
-template <int C>
-__global__ __launch_bounds__(kGT) void gk_loop_kernel(const double* K, const double* KT, int n,
+//
+// Two barriers per iteration: one after the argmax scan, one in the first
+// dot product.  Thread t owns entries t, t + T, ... of both violation
+// vectors: it alone scans and updates them, so the updates of one iteration
+// reach the next scan without a barrier.  A row update's own violation
+// viol[i1] = (new_u K[i1,:]) . v - a[i1] needs a second block sum; its wave
+// partials are left in rd2, and the next iteration sums them after its argmax
+// barrier (the owner skips i1 in its scan and the value joins the per-wave
+// pairs in the block argmax, a maximum under a total order, so the result is
+// the same).  u[i1] (v[i2]) is read before the dot-product barrier
+// and rewritten by thread 0 after it.
+template <int C, int T>
+__global__ __launch_bounds__(T) void gk_loop_kernel(const double* K, const double* KT, int n,
                                                       int m, const double* a, const double* b,
                                                       double* u, double* v, double* viol,
                                                       double* viol2, int max_iter,
                                                       double stop_thr, GkOut* out) {
-  __shared__ double rv1[kGW], rv2[kGW], rd1[kGW], rd2[kGW];
-  __shared__ int ri1[kGW], ri2[kGW];
+  constexpr int W = T / kWave;  // waves
+  static_assert(W < kWave, "one lane per wave, and one more, in the cross-wave merge");
+  __shared__ double rv1[W], rv2[W], rd1[W], rd2[W];
+  __shared__ int ri1[W], ri2[W];
   constexpr int CS = C > 0 ? C : 1;
-  constexpr int R = C > 0 ? C / kGT : 1;  // row elements per thread kept in registers
+  constexpr int R = C > 0 ? C / T : 1;  // row elements per thread kept in registers
   __shared__ double su[CS], sv[CS], sq[CS], sq2[CS];
-  const int tid = threadIdx.x, wave = tid / kWave;
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid & (kWave - 1);
   double* U = u;
   double* V = v;
   double* Q = viol;
   double* Q2 = viol2;
   if constexpr (C > 0) {
-    for (int k = tid; k < n; k += kGT) {
+    for (int k = tid; k < n; k += T) {
       su[k] = u[k];
       sq[k] = viol[k];
     }
-    for (int k = tid; k < m; k += kGT) {
+    for (int k = tid; k < m; k += T) {
       sv[k] = v[k];
       sq2[k] = viol2[k];
     }
@@ -219,18 +243,20 @@ __global__ __launch_bounds__(kGT) void gk_loop_kernel(const double* K, const dou
   }
   int ii = 0, stopped = 0;
   double stop = 0.0;
+  int pr = -1;       // the previous iteration's row update, its violation pending
+  double pa = 0.0;   // a[pr]
   for (; ii < max_iter; ++ii) {
     // ---- the two argmaxes ----
     double b1 = -__builtin_inf(), b2 = -__builtin_inf();
     int i1 = INT_MAX, i2 = INT_MAX;
-    for (int k = tid; k < n; k += kGT) {
+    for (int k = tid; k < n; k += T) {
       const double x = fabs(Q[k]);
-      if (gk_above(x, k, b1, i1)) {
+      if (k != pr && gk_above(x, k, b1, i1)) {
         b1 = x;
         i1 = k;
       }
     }
-    for (int k = tid; k < m; k += kGT) {
+    for (int k = tid; k < m; k += T) {
       const double x = fabs(Q2[k]);
       if (gk_above(x, k, b2, i2)) {
         b2 = x;
@@ -239,57 +265,61 @@ __global__ __launch_bounds__(kGT) void gk_loop_kernel(const double* K, const dou
     }
     gk_wave_argmax(b1, i1);
     gk_wave_argmax(b2, i2);
-    if ((tid & (kWave - 1)) == 0) {
+    if (lane == 0) {
       rv1[wave] = b1;
       ri1[wave] = i1;
       rv2[wave] = b2;
       ri2[wave] = i2;
     }
     __syncthreads();
-    b1 = rv1[0];
-    i1 = ri1[0];
-    b2 = rv2[0];
-    i2 = ri2[0];
-#pragma unroll
-    for (int w = 1; w < kGW; ++w) {
-      if (gk_above(rv1[w], ri1[w], b1, i1)) {
-        b1 = rv1[w];
-        i1 = ri1[w];
-      }
-      if (gk_above(rv2[w], ri2[w], b2, i2)) {
-        b2 = rv2[w];
-        i2 = ri2[w];
-      }
+    // lane w < W takes wave w's pairs, lane W the pending row's
+    b1 = b2 = -__builtin_inf();
+    i1 = i2 = INT_MAX;
+    if (lane < W) {
+      b1 = rv1[lane];
+      i1 = ri1[lane];
+      b2 = rv2[lane];
+      i2 = ri2[lane];
     }
+    if (pr >= 0) {
+      const double q = wave_sum_dpp_d(lane < W ? rd2[lane] : 0.0) - pa;
+      if (tid == pr % T) Q[pr] = q;
+      if (lane == W) {
+        b1 = fabs(q);
+        i1 = pr;
+      }
+      pr = -1;
+    }
+    gk_wave_argmax(b1, i1);
+    gk_wave_argmax(b2, i2);
     // (every thread holds the same pairs; n, m >= 1 give indexes in range)
-    i1 = __builtin_amdgcn_readfirstlane(i1);
-    i2 = __builtin_amdgcn_readfirstlane(i2);
     stop = (__builtin_isnan(b1) || __builtin_isnan(b2)) ? __builtin_nan("") : fmax(b1, b2);
     const bool row = b1 > b2;
     if (row) {
       // ---- row i1 ----
       const double* Kr = K + (int64_t)i1 * m;
       const double ai = a[i1];
+      const double u_old = U[i1];
       double s = 0.0, s2 = 0.0;
       double dot1, new_u, du;
       if constexpr (C > 0) {
         double kr[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int j = tid + r * kGT;
+          const int j = tid + r * T;
           kr[r] = j < m ? Kr[j] : 0.0;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int j = tid + r * kGT;
+          const int j = tid + r * T;
           if (j < m) s += kr[r] * V[j];
         }
-        dot1 = block_sum<kGT>(s, rd1);
+        dot1 = block_sum_dpp<W>(s, rd1);
         new_u = ai / dot1;
-        du = new_u - U[i1];
+        du = new_u - u_old;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int j = tid + r * kGT;
+          const int j = tid + r * T;
           if (j < m) {
             const double vj = V[j];
             Q2[j] += (kr[r] * du) * vj;
@@ -297,71 +327,67 @@ __global__ __launch_bounds__(kGT) void gk_loop_kernel(const double* K, const dou
           }
         }
       } else {
-        for (int j = tid; j < m; j += kGT) s += Kr[j] * V[j];
-        dot1 = block_sum<kGT>(s, rd1);
+        for (int j = tid; j < m; j += T) s += Kr[j] * V[j];
+        dot1 = block_sum_dpp<W>(s, rd1);
         new_u = ai / dot1;
-        du = new_u - U[i1];
-        for (int j = tid; j < m; j += kGT) {
+        du = new_u - u_old;
+        for (int j = tid; j < m; j += T) {
           const double kj = Kr[j], vj = V[j];
           Q2[j] += (kj * du) * vj;
           s2 += (new_u * kj) * vj;
         }
       }
-      const double dot2 = block_sum<kGT>(s2, rd2);
-      if (tid == 0) {
-        Q[i1] = dot2 - ai;
-        U[i1] = new_u;
-      }
+      s2 = wave_sum_dpp_d(s2);
+      if (lane == 0) rd2[wave] = s2;
+      if (tid == 0) U[i1] = new_u;
+      pr = i1;
+      pa = ai;
     } else {
       // ---- column i2 ----
       const double* Kc = KT + (int64_t)i2 * n;
       const double bi = b[i2];
+      const double v_old = V[i2];
       double s = 0.0;
       double dot1, new_v, dv;
       if constexpr (C > 0) {
         double kc[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int i = tid + r * kGT;
+          const int i = tid + r * T;
           kc[r] = i < n ? Kc[i] : 0.0;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int i = tid + r * kGT;
+          const int i = tid + r * T;
           if (i < n) s += kc[r] * U[i];
         }
-        dot1 = block_sum<kGT>(s, rd1);
+        dot1 = block_sum_dpp<W>(s, rd1);
         new_v = bi / dot1;
-        dv = new_v - V[i2];
+        dv = new_v - v_old;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int i = tid + r * kGT;
+          const int i = tid + r * T;
           if (i < n) Q[i] += (dv * kc[r]) * U[i];
         }
       } else {
-        for (int i = tid; i < n; i += kGT) s += Kc[i] * U[i];
-        dot1 = block_sum<kGT>(s, rd1);
+        for (int i = tid; i < n; i += T) s += Kc[i] * U[i];
+        dot1 = block_sum_dpp<W>(s, rd1);
         new_v = bi / dot1;
-        dv = new_v - V[i2];
-        for (int i = tid; i < n; i += kGT) Q[i] += (dv * Kc[i]) * U[i];
+        dv = new_v - v_old;
+        for (int i = tid; i < n; i += T) Q[i] += (dv * Kc[i]) * U[i];
       }
-      // (the same barrier count as the row update)
-      __syncthreads();
-      if (tid == 0) {
-        Q2[i2] = new_v * dot1 - bi;
-        V[i2] = new_v;
-      }
+      if (tid == i2 % T) Q2[i2] = new_v * dot1 - bi;
+      if (tid == 0) V[i2] = new_v;
     }
-    // The updates are visible to the next argmax.
-    __syncthreads();
     if (stop <= stop_thr) {
       stopped = 1;
       break;
     }
   }
   if constexpr (C > 0) {
-    for (int k = tid; k < n; k += kGT) u[k] = su[k];
-    for (int k = tid; k < m; k += kGT) v[k] = sv[k];
+    __syncthreads();
+    for (int k = tid; k < n; k += T) u[k] = su[k];
+    for (int k = tid; k < m; k += T) v[k] = sv[k];
   }
   if (tid == 0) {
     out->niter = stopped ? ii : max_iter - 1;
@@ -414,12 +440,15 @@ extern "C" int potamd_greenkhorn(int dtype, const void* M, int64_t ldm, int n, i
   HIP_TRY(hipGetLastError());
   GkOut res = {};
   if (max_iter > 0) {
-    if (n <= kGLds && m <= kGLds)
-      hipLaunchKernelGGL(gk_loop_kernel<kGLds>, dim3(1), dim3(kGT), 0, s, L.K, L.KT, n, m, a, b,
-                         u, v, L.viol, L.viol2, max_iter, stop_thr, L.out);
+    if (n <= kGSmall && m <= kGSmall)
+      hipLaunchKernelGGL((gk_loop_kernel<kGSmall, 512>), dim3(1), dim3(512), 0, s, L.K, L.KT, n,
+                         m, a, b, u, v, L.viol, L.viol2, max_iter, stop_thr, L.out);
+    else if (n <= kGLds && m <= kGLds)
+      hipLaunchKernelGGL((gk_loop_kernel<kGLds, 1024>), dim3(1), dim3(1024), 0, s, L.K, L.KT, n,
+                         m, a, b, u, v, L.viol, L.viol2, max_iter, stop_thr, L.out);
     else
-      hipLaunchKernelGGL(gk_loop_kernel<0>, dim3(1), dim3(kGT), 0, s, L.K, L.KT, n, m, a, b, u,
-                         v, L.viol, L.viol2, max_iter, stop_thr, L.out);
+      hipLaunchKernelGGL((gk_loop_kernel<0, 1024>), dim3(1), dim3(1024), 0, s, L.K, L.KT, n, m,
+                         a, b, u, v, L.viol, L.viol2, max_iter, stop_thr, L.out);
     HIP_TRY(hipGetLastError());
   }
   if (P) {

@@ -37,9 +37,11 @@ def test_greenkhorn_doctest():
                                atol=5e-9)
 
 
-# (the last case has more than 4096 rows: the loop keeps its state in HBM)
+# (up to 2048 points a side the loop runs on 512 threads, up to 4096 on 1024
+# with its state in LDS; the last case keeps its state in HBM)
 @pytest.mark.parametrize("n,m,reg,it", [(30, 40, 0.5, 20000), (64, 17, 0.2, 3000),
-                                        (200, 150, 1.0, 2000), (4200, 9, 0.5, 400)])
+                                        (200, 150, 1.0, 2000), (2048, 33, 0.5, 600),
+                                        (33, 2500, 0.5, 600), (4200, 9, 0.5, 400)])
 def test_greenkhorn_matches_oracle(n, m, reg, it):
     a, b, M = _problem(n + m, n, m)
     with warnings.catch_warnings():
