@@ -142,7 +142,7 @@ def _pow2(x):
 def _batch_key(cost, ns, ms, d):
     """Largest sizes (rounded up) and mean sizes (rounded) of the batch."""
     mean = f"{_pow2(np.mean(ns))}x{_pow2(np.mean(ms))}"
-    return f"{_arch()}|batch-{cost}|{_class(max(ns), max(ms))}|mean{mean}|d{int(d)}"
+    return f"{_arch()}|batch-{cost}|{_class(int(np.max(ns)), int(np.max(ms)))}|mean{mean}|d{int(d)}"
 
 
 def lookup_batch(cost, ns, ms, d=0):

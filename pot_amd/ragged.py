@@ -316,8 +316,9 @@ def points_from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None
     # an upload at solve time would make wait for the work already queued on
     # the stream, such as a warm start's network) go up in one copy of one
     # byte buffer, viewed on the device in their own types (no conversion
-    # kernels).
-    raw = np.empty(3 * B * 8 + 3 * B * 4, dtype=np.uint8)
+    # kernels), from pinned memory without a host wait.
+    raw_t = torch.empty(3 * B * 8 + 3 * B * 4, dtype=torch.uint8, pin_memory=True)
+    raw = raw_t.numpy()
     h64 = raw[:24 * B].view(np.int64)
     h64[0:B] = aoff
     h64[B:2 * B] = boff
@@ -326,7 +327,7 @@ def points_from_clouds(xs, ys, regs, a=None, b=None, normalize=True, device=None
     h32[0:B] = ns
     h32[B:2 * B] = ms
     h32[2 * B:3 * B] = _points_order(ns, ms, regs_h)
-    tab = torch.from_numpy(raw).to(device)
+    tab = raw_t.to(device, non_blocking=True)
     t64 = tab[:24 * B].view(torch.int64)
     t32 = tab[24 * B:].view(torch.int32)
     return RaggedPoints(device=device, ns=ns, ms=ms, d=d, X=_pack_points(xs, device),
