@@ -32,7 +32,10 @@ int potamd::device_cus() {
 
 namespace {
 
-constexpr int kRing = 4;
+// Stop checks in flight: the host runs at most kRing checks (10 iterations
+// each) ahead of the device; the iterations it queues past the device's stop
+// exit at once but still cost their launches.
+constexpr int kRing = 2;
 constexpr int kColLseRows = 256;
 
 int default_rpb(int n) {
@@ -252,11 +255,26 @@ int dense_impl(int dtype, int domain, int order, int kernel_given, const void* M
   bool ev_used[kRing] = {};
   for (int k = 0; k < kRing; ++k) HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
   int ring_pos = 0;
+  int ring_head = 0;  // oldest check whose copy the host has not read
   bool stopped_early = false;
   int rc = 0;
   const int vo = order == POTAMD_COL_FIRST ? 1 : 0;
 
   for (int ii = 0; ii < max_iter && !stopped_early; ++ii) {
+    // Checks the device has already passed end the loop without a host wait.
+    while (hc && ring_head < ring_pos) {
+      const int k = ring_head % kRing;
+      const hipError_t q = hipEventQuery(ev[k]);
+      if (q == hipErrorNotReady) break;
+      if (q != hipSuccess) {
+        rc = POTAMD_EHIP;
+        break;
+      }
+      ev_used[k] = false;
+      ++ring_head;
+      if (hc[k].done) stopped_early = true;
+    }
+    if (rc || stopped_early) break;
     ra.iter = ii;
     ra.u_old = L.U[ii & 1];
     ra.u_new = L.U[(ii + 1) & 1];
@@ -277,6 +295,8 @@ int dense_impl(int dtype, int domain, int order, int kernel_given, const void* M
       const int k = ring_pos % kRing;
       if (ev_used[k]) {
         if (hipEventSynchronize(ev[k]) != hipSuccess) { rc = POTAMD_EHIP; break; }
+        ev_used[k] = false;
+        ++ring_head;
         const Ctrl* snap = hc ? &hc[k] : &local;
         if (snap->done) stopped_early = true;
       }
