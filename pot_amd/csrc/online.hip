@@ -1300,16 +1300,31 @@ extern "C" int potamd_sinkhorn_online(int precision, int dtype, const void* X, i
   ma.pot_new = L.V[1];
   HIP_TRY(half_step(prec, D, d, SY, SX, L.U[0], L, reg, Sy, ma, s));
 
-  constexpr int kRing = 4;
+  constexpr int kRing = 2;  // stop checks in flight: the host queues at most 2 past the device
   Ctrl* hc = static_cast<Ctrl*>(host_ctrl);
   Ctrl local;
   hipEvent_t ev[kRing];
   bool ev_used[kRing] = {};
   for (int k = 0; k < kRing; ++k) HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
   int ring_pos = 0;
+  int ring_head = 0;  // oldest check whose copy the host has not read
   bool stopped = false;
   int rc = 0;
   for (int ii = 0; ii < max_iter && !stopped; ++ii) {
+    // Checks the device has already passed end the loop without a host wait.
+    while (hc && ring_head < ring_pos) {
+      const int k = ring_head % kRing;
+      const hipError_t q = hipEventQuery(ev[k]);
+      if (q == hipErrorNotReady) break;
+      if (q != hipSuccess) {
+        rc = POTAMD_EHIP;
+        break;
+      }
+      ev_used[k] = false;
+      ++ring_head;
+      if (hc[k].done) stopped = true;
+    }
+    if (rc || stopped) break;
     // u update from v_cur = V[(ii+1)&1]
     ma.check = 0;
     ma.decide = 0;
@@ -1336,6 +1351,8 @@ extern "C" int potamd_sinkhorn_online(int precision, int dtype, const void* X, i
       const int k = ring_pos % kRing;
       if (ev_used[k]) {
         if (hipEventSynchronize(ev[k]) != hipSuccess) { rc = POTAMD_EHIP; break; }
+        ev_used[k] = false;
+        ++ring_head;
         const Ctrl* snap = hc ? &hc[k] : &local;
         if (snap->done) stopped = true;
       }

@@ -456,16 +456,31 @@ int run_stabilized(const TM* M, int64_t ldm, int n, int m, int H, bool multi, do
                      alpha, beta, K, (const StabCtrl*)nullptr);
   HIP_TRY(hipGetLastError());
 
-  constexpr int kRing = 4;
+  constexpr int kRing = 2;  // stop checks in flight: the host queues at most 2 past the device
   Ctrl* hc = static_cast<Ctrl*>(host_ctrl);
   hipEvent_t ev[kRing];
   bool ev_used[kRing] = {};
   for (int k = 0; k < kRing; ++k) HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
   int ring_pos = 0;
+  int ring_head = 0;  // oldest check whose copy the host has not read
   bool stopped = false;
   int rc = 0;
   StabCtrl local;
   for (int ii = 0; ii < max_iter && !stopped; ++ii) {
+    // Checks the device has already passed end the loop without a host wait.
+    while (hc && ring_head < ring_pos) {
+      const int k = ring_head % kRing;
+      const hipError_t q = hipEventQuery(ev[k]);
+      if (q == hipErrorNotReady) break;
+      if (q != hipSuccess) {
+        rc = POTAMD_EHIP;
+        break;
+      }
+      ev_used[k] = false;
+      ++ring_head;
+      if (hc[k].done) stopped = true;
+    }
+    if (rc || stopped) break;
     hipLaunchKernelGGL(stab_copy, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, nu, u,
                        L.up, L.sc);
     hipLaunchKernelGGL(stab_copy, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, nv, v,
@@ -505,6 +520,8 @@ int run_stabilized(const TM* M, int64_t ldm, int n, int m, int H, bool multi, do
       const int k = ring_pos % kRing;
       if (ev_used[k]) {
         if (hipEventSynchronize(ev[k]) != hipSuccess) { rc = POTAMD_EHIP; break; }
+        ev_used[k] = false;
+        ++ring_head;
         const Ctrl* snap = hc ? &hc[k] : &local.c;
         if (snap->done) stopped = true;
       }
